@@ -1,0 +1,139 @@
+"""Benchmark: KKT solves/sec (+ p50/p99 latency) of the plain drop-in path on
+one MI355X, on the 10,000-bus / 13,000-branch same-pattern sequence (20 steps,
+gamma = 1e5, Ds spread growing from 1e-2..1e2 to 1e-4..1e8).
+
+One "step" is one HykktSolver::execute() of the next system of the sequence
+(numeric refactorisation + Schur CG + recovery + residual check; the symbolic
+analysis is done once, during warmup).  Inputs are resident in HBM before the
+timed region starts (inputs_on_device).  Every timed solve must reach a
+relative KKT residual <= 1e-8 over all four block rows.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def metric_name():
+    with open(os.path.join(ROOT, "BASELINE.json")) as f:
+        return json.load(f)["metric"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="grid10k_seq20")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the KKT solve of one "
+                                   "system does not shard across GPUs (see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+
+    import hykkt_amd as H
+    from hykkt_amd.generator import ds_spread_schedule, workload
+
+    g = workload(args.workload)
+    nseq = 20
+    print(f"bench: generating {args.workload} n={g.n} m={g.m} d={g.d}", file=sys.stderr,
+          flush=True)
+    dev = torch.device("cuda:0")
+    to_d = lambda a, t=torch.float64: torch.as_tensor(np.asarray(a), dtype=t, device=dev)
+    systems = []
+    for t in range(nseq):
+        rng = ds_spread_schedule(t, nseq) if args.workload == "grid10k_seq20" else None
+        sy = g.system(t, ds_range=rng)
+        systems.append(dict(H=to_d(sy.H.v), Jc=to_d(sy.Jc.v), Jd=to_d(sy.Jd.v), Ds=to_d(sy.Ds.v),
+                            rx=to_d(sy.rx), rs=to_d(sy.rs), ry=to_d(sy.ry), ryd=to_d(sy.ryd)))
+    i32 = lambda a: to_d(a, torch.int32)
+    pat = dict(Hrp=i32(g.H.rp), Hci=i32(g.H.ci), Jcrp=i32(g.Jc.rp), Jcci=i32(g.Jc.ci),
+               Jdrp=i32(g.Jd.rp), Jdci=i32(g.Jd.ci), Dsrp=i32(np.arange(g.d + 1)),
+               Dsci=i32(np.arange(g.d)))
+    torch.cuda.synchronize()
+
+    s = H.HykktSolver(g.gamma)
+    s.options(inputs_on_device=1)
+
+    def bind(k):
+        v = systems[k % nseq]
+        s.set_H(pat["Hrp"], pat["Hci"], v["H"], g.n, g.n, g.H.nnz)
+        s.set_Ds(pat["Dsrp"], pat["Dsci"], v["Ds"], g.d, g.d, g.d)
+        s.set_Jc(pat["Jcrp"], pat["Jcci"], v["Jc"], g.m, g.n, g.Jc.nnz)
+        s.set_Jd(pat["Jdrp"], pat["Jdci"], v["Jd"], g.d, g.n, g.Jd.nnz)
+        s.set_rx(v["rx"], g.n)
+        s.set_rs(v["rs"], g.d)
+        s.set_ry(v["ry"], g.m)
+        s.set_ryd(v["ryd"], g.d)
+
+    for k in range(args.warmup):
+        bind(k)
+        t0 = time.perf_counter()
+        s.execute()
+        st = s.stats()
+        print(f"bench: warmup {k} {1e3 * (time.perf_counter() - t0):.1f} ms resid4="
+              f"{st['resid4']:.2e} cg={st['cg_iters']} factor={st['t_factor_ms']:.1f}ms "
+              f"cg={st['t_cg_ms']:.1f}ms sym={st['t_symbolic_ms']:.0f}ms", file=sys.stderr,
+              flush=True)
+
+    lat, worst, iters, tf, tcg = [], 0.0, [], [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        bind(args.warmup + k)
+        t0 = time.perf_counter()
+        rc = s.execute()
+        lat.append(time.perf_counter() - t0)
+        st = s.stats()
+        worst = max(worst, st["resid4"])
+        iters.append(st["cg_iters"])
+        tf.append(st["t_factor_ms"])
+        tcg.append(st["t_cg_ms"])
+        if rc != 0 or not st["resid4"] <= 1e-8:
+            print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
+                  flush=True)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": metric_name(),
+        "value": args.steps / total,
+        "unit": "KKT solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded OPF-like generator, hykkt_amd/generator.py)",
+        "config": {"workload": f"{args.workload}: 10,000 buses / 13,000 branches, "
+                               f"n={g.n} m={g.m} d={g.d}, gamma=1e5, 20-step same-pattern "
+                               "sequence, plain drop-in path",
+                   "parallelism": "single GPU"},
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "max_resid4": worst,
+        "cg_iters_mean": float(np.mean(iters)),
+        "factor_ms_mean": float(np.mean(tf)),
+        "cg_ms_mean": float(np.mean(tcg)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,551 @@
+// capi.cpp -- extern "C" boundary of the library (declared in include/hykkt.h).
+// Exceptions never cross it: each entry point returns an error code and keeps
+// the message for hykkt_last_error().
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hykkt.h"
+#include "device.hpp"
+#include "solver.hpp"
+
+struct hykkt_solver_s {
+  hykkt::Solver impl;
+  explicit hykkt_solver_s(double g) : impl(g) {}
+};
+
+struct hykkt_chol_s {
+  hipStream_t st = nullptr;
+  std::unique_ptr<hykkt::CholEngine> eng;
+  hykkt::DevBuf<double> vals, x;
+  std::vector<int> post;
+  int n = 0, nnz = 0;
+  ~hykkt_chol_s()
+  {
+    eng.reset();
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f)
+{
+  try {
+    g_err.clear();
+    return f();
+  } catch (const hykkt::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return HYKKT_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return HYKKT_E_STATE;
+  }
+}
+
+#define NEED(p) HYKKT_REQUIRE((p) != nullptr, HYKKT_E_ARG, "null handle or pointer: " #p)
+
+}  // namespace
+
+extern "C" {
+
+const char* hykkt_version(void) { return "hykkt-mi355x 0.1.0 (gfx950)"; }
+const char* hykkt_last_error(void) { return g_err.c_str(); }
+
+int hykkt_default_options(hykkt_options* o)
+{
+  return guard([&] {
+    NEED(o);
+    o->ruiz_its = 2;
+    o->norm_tol = 1e-2;
+    o->pivot_tol = 1e-12;
+    o->cg_tol = 1e-12;
+    o->cg_itmax = 100;
+    o->delta = 0.0;
+    o->inputs_on_device = 0;
+    o->cg_check_every = 1;
+    o->verbose = 0;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_create(double gamma, hykkt_solver* out)
+{
+  return guard([&] {
+    NEED(out);
+    *out = new hykkt_solver_s(gamma);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_destroy(hykkt_solver s)
+{
+  return guard([&] {
+    delete s;
+    return HYKKT_OK;
+  });
+}
+
+#define SETBLK(name, id)                                                                   \
+  int name(hykkt_solver s, const int* rp, const int* ci, const double* v, int n, int m,   \
+           int nnz)                                                                        \
+  {                                                                                        \
+    return guard([&] {                                                                     \
+      NEED(s);                                                                             \
+      s->impl.set_block(id, rp, ci, v, n, m, nnz);                                         \
+      return HYKKT_OK;                                                                     \
+    });                                                                                    \
+  }
+SETBLK(hykkt_set_H, hykkt::BLK_H)
+SETBLK(hykkt_set_Ds, hykkt::BLK_DS)
+SETBLK(hykkt_set_Jc, hykkt::BLK_JC)
+SETBLK(hykkt_set_Jd, hykkt::BLK_JD)
+
+#define SETVEC(name, id)                                   \
+  int name(hykkt_solver s, const double* v, int n)         \
+  {                                                        \
+    return guard([&] {                                     \
+      NEED(s);                                             \
+      s->impl.set_vec(id, v, n);                           \
+      return HYKKT_OK;                                     \
+    });                                                    \
+  }
+SETVEC(hykkt_set_rx, hykkt::VEC_RX)
+SETVEC(hykkt_set_rs, hykkt::VEC_RS)
+SETVEC(hykkt_set_ry, hykkt::VEC_RY)
+SETVEC(hykkt_set_ryd, hykkt::VEC_RYD)
+SETVEC(hykkt_set_x_host, hykkt::VEC_X)
+SETVEC(hykkt_set_s_host, hykkt::VEC_S)
+SETVEC(hykkt_set_y_host, hykkt::VEC_Y)
+SETVEC(hykkt_set_yd_host, hykkt::VEC_YD)
+
+int hykkt_set_gamma(hykkt_solver s, double gamma)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.set_gamma(gamma);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_read_matrix_files(hykkt_solver s, const char* h, const char* ds, const char* jc,
+                            const char* jd, const char* rx, const char* rs, const char* ry,
+                            const char* ryd, int skip_lines)
+{
+  return guard([&] {
+    NEED(s);
+    const char* f[8] = {h, ds, jc, jd, rx, rs, ry, ryd};
+    for (const char* p : f) NEED(p);
+    s->impl.read_files(f, skip_lines);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_execute(hykkt_solver s)
+{
+  return guard([&] {
+    NEED(s);
+    return s->impl.execute();
+  });
+}
+
+int hykkt_set_options(hykkt_solver s, const hykkt_options* o)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(o);
+    HYKKT_REQUIRE(o->ruiz_its >= 0 && o->cg_itmax >= 0 && o->cg_check_every >= 0, HYKKT_E_ARG,
+                  "negative iteration count");
+    bool dev_changed = o->inputs_on_device != s->impl.opts.inputs_on_device;
+    s->impl.opts = *o;
+    (void)dev_changed;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_options(hykkt_solver s, hykkt_options* o)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(o);
+    *o = s->impl.opts;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_stats(hykkt_solver s, hykkt_stats* st)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(st);
+    *st = s->impl.stats;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_solution(hykkt_solver s, double* dx, double* ds, double* dyc, double* dyd)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.get_solution(dx, ds, dyc, dyd);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_solution_device(hykkt_solver s, const double** dx, const double** ds,
+                              const double** dyc, const double** dyd)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.get_solution_device(dx, ds, dyc, dyd);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_permutation(hykkt_solver s, int* perm, int n)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(perm);
+    s->impl.get_perm(perm, n);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_set_permutation(hykkt_solver s, const int* perm, int n)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(perm);
+    s->impl.set_perm(perm, n);
+    return HYKKT_OK;
+  });
+}
+
+long long hykkt_debug_copy(hykkt_solver s, int what, void* out, long long cap)
+{
+  long long r = 0;
+  int rc = guard([&] {
+    NEED(s);
+    NEED(out);
+    r = s->impl.debug_copy(what, out, cap);
+    return HYKKT_OK;
+  });
+  return rc == HYKKT_OK ? r : rc;
+}
+
+// ------------------------------------------------------------------ components
+int hykkt_ruiz_scale(int its, int n, int m2, const int* h_rp, const int* h_ci, double* h_v,
+                     int h_nnz, const int* jt_rp, const int* jt_ci, double* jt_v,
+                     const int* j_rp, const int* j_ci, double* j_v, int j_nnz, double* rhs1,
+                     double* rhs2, double* max_d)
+{
+  return guard([&] {
+    HYKKT_REQUIRE(n >= 0 && m2 >= 0 && h_nnz >= 0 && j_nnz >= 0 && its >= 0, HYKKT_E_ARG,
+                  "negative size");
+    NEED(h_rp);
+    NEED(jt_rp);
+    NEED(j_rp);
+    NEED(rhs1);
+    NEED(rhs2);
+    NEED(max_d);
+    using hykkt::DevBuf;
+    DevBuf<int> hrp, hci, jtrp, jtci, jrp, jci;
+    DevBuf<double> hv, jtv, jv, r1, r2, md, sc;
+    auto upi = [](DevBuf<int>& b, const int* p, int cnt) {
+      std::vector<int> v(p, p + cnt);
+      if (v.empty()) v.push_back(0);
+      b.upload(v);
+    };
+    auto upd = [](DevBuf<double>& b, const double* p, int cnt) {
+      std::vector<double> v(p, p + cnt);
+      if (v.empty()) v.push_back(0.0);
+      b.upload(v);
+    };
+    upi(hrp, h_rp, n + 1);
+    upi(hci, h_ci, h_nnz);
+    upd(hv, h_v, h_nnz);
+    upi(jtrp, jt_rp, n + 1);
+    upi(jtci, jt_ci, j_nnz);
+    upd(jtv, jt_v, j_nnz);
+    upi(jrp, j_rp, m2 + 1);
+    upi(jci, j_ci, j_nnz);
+    upd(jv, j_v, j_nnz);
+    upd(r1, rhs1, n);
+    upd(r2, rhs2, m2);
+    md.alloc(std::max(1, n + m2));
+    sc.alloc(std::max(1, n + m2));
+    hipStream_t st = nullptr;
+    if (its == 0) hykkt::dev::fill(md.p, 1.0, n + m2, st);
+    for (int it = 0; it < its; it++) {
+      hykkt::dev::ruiz_rowmax(n, n + m2, hrp.p, hv.p, jtrp.p, nullptr, jtv.p, jrp.p, jv.p, sc.p, st);
+      hykkt::dev::ruiz_scale(n, n + m2, hrp.p, hci.p, hv.p, jtrp.p, jtci.p, jtv.p, jrp.p, jci.p,
+                             jv.p, sc.p, r1.p, r2.p, md.p, it == 0, st);
+    }
+    HIPCHK(hipDeviceSynchronize());
+    auto down = [](double* dst, const DevBuf<double>& b, int cnt) {
+      if (cnt) HIPCHK(hipMemcpy(dst, b.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    };
+    down(h_v, hv, h_nnz);
+    down(jt_v, jtv, j_nnz);
+    down(j_v, jv, j_nnz);
+    down(rhs1, r1, n);
+    down(rhs2, r2, m2);
+    down(max_d, md, n + m2);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_perm_reverse(int n, const int* perm, int* rev)
+{
+  return guard([&] {
+    NEED(perm);
+    NEED(rev);
+    std::vector<int> p(perm, perm + n), r;
+    hykkt::perm_reverse(p, r);
+    std::memcpy(rev, r.data(), sizeof(int) * n);
+    return HYKKT_OK;
+  });
+}
+
+static hykkt::Pattern make_pattern(int n, int m, const int* rows, const int* cols)
+{
+  hykkt::Pattern a;
+  a.n = n;
+  a.m = m;
+  a.rp.assign(rows, rows + n + 1);
+  a.ci.assign(cols, cols + rows[n]);
+  return a;
+}
+
+int hykkt_perm_map_rc(int n, const int* rows, const int* cols, const int* perm, const int* rev,
+                      int* prows, int* pcols, int* pmap)
+{
+  return guard([&] {
+    NEED(rows);
+    NEED(perm);
+    NEED(rev);
+    hykkt::Pattern a = make_pattern(n, n, rows, cols), pa;
+    std::vector<int> p(perm, perm + n), r(rev, rev + n), map;
+    hykkt::perm_map_rc(a, p, r, pa, map);
+    std::memcpy(prows, pa.rp.data(), sizeof(int) * (n + 1));
+    std::memcpy(pcols, pa.ci.data(), sizeof(int) * pa.ci.size());
+    std::memcpy(pmap, map.data(), sizeof(int) * map.size());
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_perm_map_c(int n, const int* rows, const int* cols, const int* rev, int* pcols, int* pmap)
+{
+  return guard([&] {
+    NEED(rows);
+    NEED(rev);
+    int mcols = 0;
+    for (int q = 0; q < rows[n]; q++) mcols = std::max(mcols, cols[q] + 1);
+    hykkt::Pattern a = make_pattern(n, mcols, rows, cols), pa;
+    std::vector<int> r(rev, rev + mcols), map;
+    hykkt::perm_map_c(a, r, pa, map);
+    std::memcpy(pcols, pa.ci.data(), sizeof(int) * pa.ci.size());
+    std::memcpy(pmap, map.data(), sizeof(int) * map.size());
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_perm_map_r(int n, const int* rows, const int* cols, const int* perm, int* prows,
+                     int* pcols, int* pmap)
+{
+  return guard([&] {
+    NEED(rows);
+    NEED(perm);
+    hykkt::Pattern a = make_pattern(n, 0, rows, cols), pa;
+    std::vector<int> p(perm, perm + n), map;
+    hykkt::perm_map_r(a, p, pa, map);
+    std::memcpy(prows, pa.rp.data(), sizeof(int) * (n + 1));
+    std::memcpy(pcols, pa.ci.data(), sizeof(int) * pa.ci.size());
+    std::memcpy(pmap, map.data(), sizeof(int) * map.size());
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_amd_order(int n, const int* rp, const int* ci, int* perm)
+{
+  return guard([&] {
+    NEED(rp);
+    NEED(perm);
+    hykkt::Pattern a = make_pattern(n, n, rp, ci);
+    std::vector<int> p = hykkt::amd_order(a);
+    std::memcpy(perm, p.data(), sizeof(int) * n);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_chol_analyze_host(int n, const int* rp, const int* ci, long long* nnz_l,
+                            double* flops, int* ns, int* nlev, int* max_front)
+{
+  return guard([&] {
+    NEED(rp);
+    hykkt::Pattern a = make_pattern(n, n, rp, ci);
+    hykkt::CholPlan p = hykkt::chol_analyze(a, hykkt::CholOptions{});
+    if (nnz_l) *nnz_l = p.nnz_l_true;
+    if (flops) *flops = p.flops;
+    if (ns) *ns = p.ns;
+    if (nlev) *nlev = static_cast<int>(p.level_ptr.size()) - 1;
+    if (max_front) *max_front = p.max_front;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_chol_create(int n, int nnz, const int* rp, const int* ci, hykkt_chol* out)
+{
+  return guard([&] {
+    NEED(rp);
+    NEED(out);
+    HYKKT_REQUIRE(rp[n] == nnz, HYKKT_E_ARG, "rowptr[n] != nnz");
+    auto c = std::make_unique<hykkt_chol_s>();
+    HIPCHK(hipStreamCreate(&c->st));
+    hykkt::Pattern a = make_pattern(n, n, rp, ci);
+    c->eng = std::make_unique<hykkt::CholEngine>(a, hykkt::CholOptions{}, c->st);
+    c->vals.alloc(std::max(1, nnz));
+    c->x.alloc(std::max(1, n));
+    c->post = c->eng->plan().post;
+    c->n = n;
+    c->nnz = nnz;
+    *out = c.release();
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_chol_factor(hykkt_chol c, const double* vals, double tol, int* singularity)
+{
+  return guard([&] {
+    NEED(c);
+    NEED(vals);
+    if (c->nnz)
+      HIPCHK(hipMemcpyAsync(c->vals.p, vals, c->nnz * sizeof(double), hipMemcpyHostToDevice, c->st));
+    c->eng->assemble(c->vals.p);
+    c->eng->factor(tol);
+    int piv = c->eng->pivot();
+    if (singularity) *singularity = piv;
+    return piv >= 0 ? HYKKT_E_PIVOT : HYKKT_OK;
+  });
+}
+
+int hykkt_chol_solve(hykkt_chol c, const double* b, double* x)
+{
+  return guard([&] {
+    NEED(c);
+    NEED(b);
+    NEED(x);
+    const int n = c->n;
+    std::vector<double> t(n);
+    for (int k = 0; k < n; k++) t[k] = b[c->post[k]];
+    HIPCHK(hipMemcpyAsync(c->x.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    c->eng->solve(c->x.p);
+    HIPCHK(hipMemcpyAsync(t.data(), c->x.p, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < n; k++) x[c->post[k]] = t[k];
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_chol_info(hykkt_chol c, long long* nnz_l, int* ns, int* nlev, double* flops)
+{
+  return guard([&] {
+    NEED(c);
+    const hykkt::CholPlan& p = c->eng->plan();
+    if (nnz_l) *nnz_l = p.nnz_l_true;
+    if (ns) *ns = p.ns;
+    if (nlev) *nlev = static_cast<int>(p.level_ptr.size()) - 1;
+    if (flops) *flops = p.flops;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_chol_destroy(hykkt_chol c)
+{
+  return guard([&] {
+    delete c;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_schur_cg(hykkt_chol c, int m, int n, const int* j_rp, const int* j_ci,
+                   const double* j_v, double* x, const double* b, double tol, int itmax,
+                   int* iters, double* final_norm)
+{
+  return guard([&] {
+    NEED(c);
+    NEED(j_rp);
+    NEED(x);
+    NEED(b);
+    HYKKT_REQUIRE(n == c->n, HYKKT_E_ARG, "J must have as many columns as the factor");
+    using hykkt::DevBuf;
+    hykkt::Pattern J = make_pattern(m, n, j_rp, j_ci), Jh;
+    std::vector<int> ipos(n), jmap;
+    for (int k = 0; k < n; k++) ipos[c->post[k]] = k;
+    hykkt::perm_map_c(J, ipos, Jh, jmap);
+    hykkt::TransposeMap jt = hykkt::transpose_map(Jh);
+    std::vector<double> jhv(jmap.size()), jthv(jt.map.size());
+    for (size_t q = 0; q < jmap.size(); q++) jhv[q] = j_v[jmap[q]];
+    for (size_t q = 0; q < jt.map.size(); q++) jthv[q] = jhv[jt.map[q]];
+    DevBuf<int> jrp, jci, jtrp, jtci;
+    DevBuf<double> jv, jtv, t1, xv, r, w, p, s, bb, sc, part;
+    auto upi = [](DevBuf<int>& d, const std::vector<int>& v) {
+      if (v.empty()) d.upload(std::vector<int>{0});
+      else d.upload(v);
+    };
+    auto upd = [](DevBuf<double>& d, const std::vector<double>& v) {
+      if (v.empty()) d.upload(std::vector<double>{0.0});
+      else d.upload(v);
+    };
+    upi(jrp, Jh.rp);
+    upi(jci, Jh.ci);
+    upd(jv, jhv);
+    upi(jtrp, jt.at.rp);
+    upi(jtci, jt.at.ci);
+    upd(jtv, jthv);
+    upd(xv, std::vector<double>(x, x + m));
+    upd(bb, std::vector<double>(b, b + m));
+    t1.alloc(std::max(1, n));
+    for (DevBuf<double>* q : {&r, &w, &p, &s}) q->alloc(std::max(1, m));
+    sc.alloc(8);
+    part.alloc(hykkt::dev::reduce_partials());
+    hykkt::SchurOps o;
+    o.m = m;
+    o.n = n;
+    o.jh_rp = jrp.p;
+    o.jh_ci = jci.p;
+    o.jh_v = jv.p;
+    o.jth_rp = jtrp.p;
+    o.jth_ci = jtci.p;
+    o.jth_v = jtv.p;
+    o.chol = c->eng.get();
+    o.st = c->st;
+    o.t1 = t1.p;
+    o.x = xv.p;
+    o.r = r.p;
+    o.w = w.p;
+    o.p = p.p;
+    o.s = s.p;
+    o.sc = sc.p;
+    o.part = part.p;
+    int it = 0;
+    double fn = 0.0;
+    int rc = hykkt::schur_cg_run(o, bb.p, tol, itmax, 1, &it, &fn);
+    HIPCHK(hipMemcpyAsync(x, xv.p, m * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (iters) *iters = it;
+    if (final_norm) *final_norm = fn;
+    return rc;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// cg.cpp -- Schur-complement conjugate gradient on the device.
+//
+// Same recurrences as SchurComplementConjugateGradient.cpp:102-194
+// (Chronopoulos-Gear form: one S-apply and two dots per iteration):
+//   p = beta p + r;  s = beta s + w;  x += alpha p;  r -= alpha s;
+//   gam1 = r.r;  stop if sqrt(gam1) < tol;  w = S r;  delta = w.r;
+//   beta = gam1/gam;  gam = gam1;  alpha = gam/(delta - beta gam/alpha).
+// Scalars stay on the device; only gam1 is read back for the stopping test.
+#include <algorithm>
+#include <cmath>
+
+#include "device.hpp"
+
+namespace hykkt {
+
+namespace {
+
+// w = S v = Jh A^{-1} Jh^T v  (beta_out = 0) or out = alpha*S v + beta*y_in
+void apply_s(SchurOps& o, const double* v, double alpha, double beta, const double* y_in,
+             double* out)
+{
+  dev::spmv(o.n, o.jth_rp, o.jth_ci, nullptr, o.jth_v, 1.0, v, 0.0, nullptr, o.t1, o.st);
+  o.chol->solve(o.t1);
+  dev::spmv(o.m, o.jh_rp, o.jh_ci, nullptr, o.jh_v, alpha, o.t1, beta, y_in, out, o.st);
+}
+
+}  // namespace
+
+int schur_cg_run(SchurOps& o, const double* b, double tol, int itmax, int check_every,
+                 int* iters, double* final_norm)
+{
+  const int m = o.m;
+  double* sc = o.sc;
+  // setup(): p = s = b (:89-100); r = b - S x0 (:104-122)
+  dev::copy(o.p, b, m, o.st);
+  dev::copy(o.s, b, m, o.st);
+  apply_s(o, o.x, -1.0, 1.0, b, o.r);
+  dev::dot(m, o.r, o.r, o.part, sc + 0, o.st);
+  double h = 0.0;
+  HIPCHK(hipMemcpyAsync(&h, sc + 0, sizeof(double), hipMemcpyDeviceToHost, o.st));
+  HIPCHK(hipStreamSynchronize(o.st));
+  double gam1 = h;
+  *iters = 0;
+  if (h == 0.0) {
+    // x0 already solves the system; the reference would divide 0/0 here.
+    *final_norm = 0.0;
+    return 0;
+  }
+  // w = S r, delta = w.r, alpha = gam/delta (:124-144)
+  apply_s(o, o.r, 1.0, 0.0, nullptr, o.w);
+  dev::dot(m, o.w, o.r, o.part, sc + 2, o.st);
+  dev::cg_scalars(sc, 1, o.st);
+  const int every = std::max(1, check_every);
+  int i = 0;
+  for (; i < itmax; i++) {
+    dev::cg_update(m, sc, o.p, o.s, o.x, o.r, o.w, o.st);
+    dev::dot(m, o.r, o.r, o.part, sc + 1, o.st);
+    if ((i + 1) % every == 0 || i + 1 == itmax) {
+      HIPCHK(hipMemcpyAsync(&h, sc + 1, sizeof(double), hipMemcpyDeviceToHost, o.st));
+      HIPCHK(hipStreamSynchronize(o.st));
+      gam1 = h;
+      if (std::sqrt(gam1) < tol) break;
+    }
+    apply_s(o, o.r, 1.0, 0.0, nullptr, o.w);
+    dev::dot(m, o.w, o.r, o.part, sc + 2, o.st);
+    dev::cg_scalars(sc, 0, o.st);
+  }
+  *iters = (i < itmax) ? i + 1 : itmax;
+  *final_norm = std::sqrt(gam1);
+  return (i == itmax) ? 1 : 0;
+}
+
+}  // namespace hykkt

@@ -1,0 +1,125 @@
+// device.hpp -- RAII device buffers and the supernodal Cholesky engine.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "symbolic.hpp"
+
+namespace hykkt {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t cnt)
+  {
+    release();
+    if (cnt) HIPCHK(hipMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(T)));
+    n = cnt;
+  }
+  void ensure(size_t cnt)
+  {
+    if (cnt > n) alloc(cnt);
+  }
+  void upload(const std::vector<T>& v)
+  {
+    alloc(v.size());
+    if (n) HIPCHK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const T* v, size_t cnt)
+  {
+    alloc(cnt);
+    if (n) HIPCHK(hipMemcpy(p, v, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Numeric supernodal multifrontal Cholesky on the device for one sparsity
+// pattern (the analysis is done once; factor() and solve() run per solve).
+class CholEngine {
+ public:
+  // `a`: full n x n CSR whose lower triangle is factored.
+  CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st);
+  const CholPlan& plan() const { return pl_; }
+  double* L() { return L_.p; }
+  // Zero the factor storage (before assembly of the matrix entries).
+  void zero_L();
+  // Scatter values (in a's CSR order) into the factor storage (zeroes first).
+  void assemble(const double* vals_dev);
+  // Add `delta` to every diagonal entry of the assembled matrix.
+  void shift_diag(double delta);
+  // Factor the assembled matrix; pivot failures are recorded on the device.
+  void factor(double pivot_tol);
+  // First failing pivot (internal index) or -1; synchronizes the stream.
+  int pivot();
+  // Solve L L^T x = b in place, x in internal (postordered) order.
+  void solve(double* x);
+  // Device arrays of the assembly map (a's lower-triangle entries).
+  const int* a_src() const { return a_src_.p; }
+  const int64_t* a_dst() const { return a_dst_.p; }
+  int n_assembled() const { return static_cast<int>(pl_.a_src.size()); }
+  const int* post_dev() const { return post_.p; }
+
+ private:
+  struct Launch {
+    int off = 0, cnt = 0;
+  };
+  struct Level {
+    Launch sns, ea_panel, ea_u, syrk;
+    std::vector<Launch> potrf, trsm, upd;
+  };
+  void build_tasks();
+
+  CholPlan pl_;
+  hipStream_t st_;
+  dev::CholDev cd_{};
+  DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_;
+  DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_;
+  DevBuf<int> a_src_;
+  DevBuf<double> L_, U_, ubuf_, wbuf_;
+  DevBuf<int> pivot_;
+  // task lists
+  std::vector<Level> lv_;
+  DevBuf<int> t_sns_, t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
+  DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tile_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
+  DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
+  int max_level_width_ = 0;
+};
+
+// The Schur-complement operator S = Jh A^{-1} Jh^T with A factored by `chol`
+// (Jh: m x n, columns in the factor's internal order) and the CG workspace.
+struct SchurOps {
+  int m = 0, n = 0;
+  const int *jh_rp = nullptr, *jh_ci = nullptr;
+  const double* jh_v = nullptr;
+  const int *jth_rp = nullptr, *jth_ci = nullptr;
+  const double* jth_v = nullptr;
+  CholEngine* chol = nullptr;
+  hipStream_t st = nullptr;
+  double *t1 = nullptr;                                            // n
+  double *x = nullptr, *r = nullptr, *w = nullptr, *p = nullptr, *s = nullptr;   // m
+  double *sc = nullptr, *part = nullptr;
+};
+
+// SchurComplementConjugateGradient::solve (…ConjugateGradient.cpp:89-194) on
+// the device: ops.x holds x0 on entry and the solution on exit.  The host
+// reads ||r||^2 back every `check_every` iterations for the stopping test.
+// Returns 0 if converged (sqrt(r.r) < tol), 1 after itmax iterations.
+int schur_cg_run(SchurOps& ops, const double* b, double tol, int itmax, int check_every,
+                 int* iters, double* final_norm);
+
+}  // namespace hykkt

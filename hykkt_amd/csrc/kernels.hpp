@@ -1,0 +1,134 @@
+// kernels.hpp -- launchers for the gfx950 HIP kernels (device pointers only).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace hykkt {
+namespace dev {
+
+void check(hipError_t e, const char* what);
+#define HIPCHK(x) ::hykkt::dev::check((x), #x)
+
+// ---------------------------------------------------------------- vectors
+void fill(double* p, double v, int64_t n, hipStream_t st);
+void fill_i(int* p, int v, int64_t n, hipStream_t st);
+void copy(double* dst, const double* src, int64_t n, hipStream_t st);
+// dst[i] = src[map[i]]  (map_idx, permcheck.cu:12-19)
+void gather(double* dst, const double* src, const int* map, int64_t n, hipStream_t st);
+// dst[i] = src[map[i]] * scale[i]
+void gather_scale(double* dst, const double* src, const int* map, const double* scale, int n,
+                  hipStream_t st);
+// y[i] *= s[i]  (vec_scale, matrix_vector_ops.cu:385-392)
+void vec_scale(double* y, const double* s, int n, hipStream_t st);
+
+// ---------------------------------------------------------------- sparse
+// jd_vs = Ds-row-scaled Jd; ryd_s = ryd*ds + rs  (row_scale :490-509, add_vecs :310-317)
+void row_scale(int d, const int* jd_rp, const double* jd_v, const double* ds,
+               const double* ryd, const double* rs, double* jd_vs, double* ryd_s, hipStream_t st);
+// y_out[i] = alpha*t + beta*y_in[i] (beta == 0: alpha*t), t = sum fma(v[map?map[q]:q], x[ci[q]])
+void spmv(int nrows, const int* rp, const int* ci, const int* map, const double* v, double alpha,
+          const double* x, double beta, const double* y_in, double* y_out, hipStream_t st);
+// out[p] = d_v[d_src[p]] (or 0) + alpha_p*acc, acc = sum fma(av[ta], bv[tb]) over terms of p
+void spgemm_values(int nnz_out, const int* d_src, const double* d_v, const int* term_ptr,
+                   const int* ta, const int* tb, const double* av, const double* bv,
+                   double alpha_p, double* out, hipStream_t st);
+// out[ent_dst[e]] = value of entry ent_src[e] (as spgemm_values) [+ delta if diag[e]]
+void spgemm_scatter(int nent, const int* ent_src, const int64_t* ent_dst, const int* d_src,
+                    const double* d_v, const int* term_ptr, const int* ta, const int* tb,
+                    const double* av, const double* bv, double alpha_p, double* out,
+                    hipStream_t st);
+// out[dst[e]] = vals[src[e]] for e < n
+void scatter(double* out, const int64_t* dst, const double* vals, const int* src, int n,
+             hipStream_t st);
+// add `delta` at out[pos[i]] for i < n (add_diag hook, matrix_vector_ops.cu:344-358)
+void add_at(double* out, const int64_t* pos, int n, double delta, hipStream_t st);
+
+// ---------------------------------------------------------------- Ruiz
+// adapt_row_max (matrix_vector_ops.cu:218-266). Jt rows read as jt_v[jt_map ? jt_map[q] : q].
+void ruiz_rowmax(int n, int totn, const int* h_rp, const double* h_v, const int* jt_rp,
+                 const int* jt_map, const double* jt_v, const int* j_rp, const double* j_v,
+                 double* scale, hipStream_t st);
+// adapt_diag_scale (matrix_vector_ops.cu:146-186); jt_v_scaled may be null (Jt shares J).
+// first != 0: max_d is written as 1*scale (reset, :268-283, fused).
+void ruiz_scale(int n, int totn, const int* h_rp, const int* h_ci, double* h_v,
+                const int* jt_rp, const int* jt_ci, double* jt_v_scaled,
+                const int* j_rp, const int* j_ci, double* j_v, const double* scale,
+                double* rhs1, double* rhs2, double* max_d, int first, hipStream_t st);
+
+// ---------------------------------------------------------------- reductions
+// Deterministic sum of a[i]*b[i] (b == null: sum of a[i]); result to *out.
+// `partials` needs reduce_partials() doubles.
+int reduce_partials();
+void dot(int n, const double* a, const double* b, double* partials, double* out, hipStream_t st);
+
+// ---------------------------------------------------------------- CG
+// CG scalars live in a device array: [0]=gam [1]=gam1 [2]=delta [3]=alpha [4]=beta
+// p = beta p + r; s = beta s + w; x += alpha p; r -= alpha s
+void cg_update(int m, const double* sc, double* p, double* s, double* x, double* r,
+               const double* w, hipStream_t st);
+// alpha = gam/delta (init=1) or beta = gam1/gam; gam = gam1; alpha = gam/(delta - beta*gam/alpha)
+void cg_scalars(double* sc, int init, hipStream_t st);
+
+// ---------------------------------------------------------------- recover / residual
+// Residual pieces, squares written per element for dot():
+//   n rows: r1 = rx - H dx - Jc^T dyc - Jd^T dyd   (Jc^T, Jd^T via transposes + maps)
+void resid_rows_n(int n, const int* h_rp, const int* h_ci, const double* h_v,
+                  const int* jct_rp, const int* jct_ci, const int* jct_map, const double* jc_v,
+                  const int* jdt_rp, const int* jdt_ci, const int* jdt_map, const double* jd_v,
+                  const double* rx, const double* dx, const double* dyc, const double* dyd,
+                  double* sq, hipStream_t st);
+void resid_rows_m(int m, const int* jc_rp, const int* jc_ci, const double* jc_v, const double* ry,
+                  const double* dx, double* sq, hipStream_t st);
+void resid_rows_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, int has_jd,
+                  const double* ds, const double* rs, const double* ryd, const double* dx,
+                  const double* dsv, const double* dyd, double* sq2, double* sq4, hipStream_t st);
+// ds = Jd dx - ryd (or -ryd); dyd = ds*Ds - rs
+void recover_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, int has_jd,
+               const double* dx, const double* ryd, const double* ds_diag, const double* rs,
+               double* dsv, double* dyd, hipStream_t st);
+
+// ---------------------------------------------------------------- Cholesky
+struct CholDev {
+  int n, ns;
+  const int* sn_first;      // ns+1
+  const int* rows_ptr;      // ns+1
+  const int* rows;
+  const int* rel;
+  const int64_t* l_off;     // ns+1
+  const int64_t* u_off;     // ns+1
+  const int* child_ptr;
+  const int* child_list;
+  double* L;
+  double* U;
+  int* pivot;               // device int: first failing pivot (min), INT_MAX if none
+  double pivot_tol;
+};
+
+// Extend-add tasks: task t = (parent p, parent front column col);
+// contributions [cptr[t], cptr[t+1]) are (child, child-local column) pairs.
+void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task_col,
+                const int* cptr, const int* contrib_child, const int* contrib_col, hipStream_t st);
+// Block step `step` of the panel factorization for the supernodes in `sns`.
+void potrf_block(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st);
+// TRSM tasks (supernode, 64-row tile index below the diagonal block of `step`).
+void trsm_block(const CholDev& c, int ntasks, const int* task_sn, const int* task_tile,
+                int step, hipStream_t st);
+// Trailing panel update tasks (supernode, tile row, tile col), K = block `step`.
+void panel_update(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
+                  const int* task_tc, int step, hipStream_t st);
+// U = -L21 L21^T tasks (supernode, tile row, tile col) over the nr x nr lower part.
+void syrk_update(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
+                 const int* task_tc, hipStream_t st);
+
+// Level-scheduled supernodal solves of L L^T x = b in place on x (internal
+// order).  ubuf holds the forward update vectors (sum of nr doubles).
+// wbuf: scratch of nsn * max_front doubles.
+void trsv_forward(const CholDev& c, int nsn, const int* sns, double* x, double* ubuf,
+                  const int64_t* uvec_off, double* wbuf, int max_front, hipStream_t st);
+void trsv_backward(const CholDev& c, int nsn, const int* sns, double* x, double* wbuf,
+                   int max_front, hipStream_t st);
+
+}  // namespace dev
+}  // namespace hykkt

@@ -42,8 +42,6 @@ def main():
                                    "system does not shard across GPUs (see DESIGN.md)"}))
         sys.exit(2)
 
-    import torch
-
     import hykkt_amd as H
     from hykkt_amd.generator import ds_spread_schedule, workload
 
@@ -51,19 +49,18 @@ def main():
     nseq = 20
     print(f"bench: generating {args.workload} n={g.n} m={g.m} d={g.d}", file=sys.stderr,
           flush=True)
-    dev = torch.device("cuda:0")
-    to_d = lambda a, t=torch.float64: torch.as_tensor(np.asarray(a), dtype=t, device=dev)
+    to_d = lambda a, t=np.float64: H.DeviceArray(np.asarray(a, dtype=t))
     systems = []
     for t in range(nseq):
         rng = ds_spread_schedule(t, nseq) if args.workload == "grid10k_seq20" else None
         sy = g.system(t, ds_range=rng)
         systems.append(dict(H=to_d(sy.H.v), Jc=to_d(sy.Jc.v), Jd=to_d(sy.Jd.v), Ds=to_d(sy.Ds.v),
                             rx=to_d(sy.rx), rs=to_d(sy.rs), ry=to_d(sy.ry), ryd=to_d(sy.ryd)))
-    i32 = lambda a: to_d(a, torch.int32)
+    i32 = lambda a: to_d(a, np.int32)
     pat = dict(Hrp=i32(g.H.rp), Hci=i32(g.H.ci), Jcrp=i32(g.Jc.rp), Jcci=i32(g.Jc.ci),
                Jdrp=i32(g.Jd.rp), Jdci=i32(g.Jd.ci), Dsrp=i32(np.arange(g.d + 1)),
                Dsci=i32(np.arange(g.d)))
-    torch.cuda.synchronize()
+    H.device_synchronize()
 
     s = H.HykktSolver(g.gamma)
     s.options(inputs_on_device=1)
@@ -90,7 +87,7 @@ def main():
               flush=True)
 
     lat, worst, iters, tf, tcg = [], 0.0, [], [], []
-    torch.cuda.synchronize()
+    H.device_synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
         bind(args.warmup + k)
@@ -105,7 +102,7 @@ def main():
         if rc != 0 or not st["resid4"] <= 1e-8:
             print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
                   flush=True)
-    torch.cuda.synchronize()
+    H.device_synchronize()
     total = time.perf_counter() - t_start
     lat_ms = np.array(lat) * 1e3
     out = {

@@ -90,6 +90,11 @@ _SIGS = {
     "hykkt_get_permutation": ([_vp, _vp, C.c_int], C.c_int),
     "hykkt_set_permutation": ([_vp, _vp, C.c_int], C.c_int),
     "hykkt_debug_copy": ([_vp, C.c_int, _vp, C.c_longlong], C.c_longlong),
+    "hykkt_device_malloc": ([C.POINTER(_vp), C.c_ulonglong], C.c_int),
+    "hykkt_device_free": ([_vp], C.c_int),
+    "hykkt_memcpy_htod": ([_vp, _vp, C.c_ulonglong], C.c_int),
+    "hykkt_memcpy_dtoh": ([_vp, _vp, C.c_ulonglong], C.c_int),
+    "hykkt_device_synchronize": ([], C.c_int),
     "hykkt_ruiz_scale": ([C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                           _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int),
     "hykkt_perm_reverse": ([C.c_int, _vp, _vp], C.c_int),
@@ -141,8 +146,8 @@ def _p(a):
         return None
     if isinstance(a, int):
         return a
-    if hasattr(a, "data_ptr"):          # torch tensor (device memory)
-        return a.data_ptr()
+    if isinstance(a, DeviceArray):
+        return a.ptr
     return a.ctypes.data
 
 
@@ -158,7 +163,7 @@ class HykktSolver:
     """Mirror of the reference's HykktSolver (src/HykktSolver.hpp:17-137).
 
     Setter arguments are numpy arrays (host) or, with
-    ``options(inputs_on_device=1)``, device tensors / raw device addresses.
+    ``options(inputs_on_device=1)``, DeviceArray objects / raw device addresses.
     The solver keeps references to the arrays it was given (they are
     borrowed, HykktSolver.cpp:932-959)."""
 
@@ -283,6 +288,39 @@ class HykktSolver:
         if r < 0:
             raise HykktError(int(r), lib().hykkt_last_error().decode())
         return out[:r]
+
+
+class DeviceArray:
+    """A numpy array copied into device memory (HBM) through the library's own
+    HIP runtime; `.ptr` is the device address to pass with inputs_on_device."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.dtype, self.shape, self.nbytes = a.dtype, a.shape, a.nbytes
+        p = C.c_void_p()
+        _check(lib().hykkt_device_malloc(C.byref(p), max(a.nbytes, 8)))
+        self.ptr = p.value
+        _check(lib().hykkt_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().hykkt_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().hykkt_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    _check(lib().hykkt_device_synchronize())
 
 
 # ---------------------------------------------------------------- components

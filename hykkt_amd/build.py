@@ -24,6 +24,8 @@ HIP_SOURCES = {
     # bit-identical to the CPU oracle's operation order
     "kernels_basic.hip": ["-ffp-contract=off"],
     "kernels_chol.hip": [],
+    "kernels_dense.hip": [],
+    "kernels_trsv.hip": [],
 }
 CXX_SOURCES = ["symbolic.cpp", "amd.cpp", "cholplan.cpp", "mmio.cpp", "chol_engine.cpp",
                "cg.cpp", "solver.cpp", "capi.cpp"]

@@ -242,6 +242,48 @@ long long hykkt_debug_copy(hykkt_solver s, int what, void* out, long long cap)
   return rc == HYKKT_OK ? r : rc;
 }
 
+int hykkt_device_malloc(void** ptr, unsigned long long bytes)
+{
+  return guard([&] {
+    NEED(ptr);
+    *ptr = nullptr;
+    if (bytes) HIPCHK(hipMalloc(ptr, bytes));
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_device_free(void* ptr)
+{
+  return guard([&] {
+    if (ptr) HIPCHK(hipFree(ptr));
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_memcpy_htod(void* dst, const void* src, unsigned long long bytes)
+{
+  return guard([&] {
+    if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes)
+{
+  return guard([&] {
+    if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_device_synchronize(void)
+{
+  return guard([&] {
+    HIPCHK(hipDeviceSynchronize());
+    return HYKKT_OK;
+  });
+}
+
 // ------------------------------------------------------------------ components
 int hykkt_ruiz_scale(int its, int n, int m2, const int* h_rp, const int* h_ci, double* h_v,
                      int h_nnz, const int* jt_rp, const int* jt_ci, double* jt_v,
@@ -451,6 +493,8 @@ int hykkt_chol_solve(hykkt_chol c, const double* b, double* x)
     c->eng->solve(c->x.p);
     HIPCHK(hipMemcpyAsync(t.data(), c->x.p, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    HYKKT_REQUIRE(!c->eng->solve_error(), HYKKT_E_STATE,
+                  "triangular-solve sweep exceeded its wait bound (results invalid)");
     for (int k = 0; k < n; k++) x[c->post[k]] = t[k];
     return HYKKT_OK;
   });

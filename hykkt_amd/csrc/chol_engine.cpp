@@ -1,8 +1,14 @@
-// chol_engine.cpp -- host orchestration of the device multifrontal Cholesky:
-// builds the per-level task lists once per pattern, then launches
-// assembly -> (extend-add, blocked panel factorization, update-matrix SYRK,
-// extend-add of update matrices) per tree level, and the level-scheduled
-// supernodal triangular solves.
+// chol_engine.cpp -- host orchestration of the device multifrontal Cholesky.
+//
+// Numeric factorization, per tree level (leaves first):
+//   extend-add of the children's update matrices into the panels,
+//   blocked right-looking panel factorization in 128-column steps
+//     (factor + invert the diagonal block; TRSM of the rows below with the
+//      inverted block on the matrix cores; MFMA update of the panel's trailing
+//      columns),
+//   U = -L21 L21^T on the matrix cores,
+//   extend-add of the children's update matrices into U.
+// Solves: one forward and one backward sweep over the whole tree.
 #include <algorithm>
 #include <climits>
 
@@ -11,26 +17,30 @@
 namespace hykkt {
 
 namespace {
-constexpr int NB = 64;
-constexpr int TS = 64;
+constexpr int NB = 128;   // panel step, MFMA tile edge, solve block
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
 CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st) : st_(st)
 {
   pl_ = chol_analyze(a, opt);
-  sn_first_.upload(pl_.sn_first);
-  rows_ptr_.upload(pl_.rows_ptr);
-  rows_.upload(pl_.rows.empty() ? std::vector<int>{0} : pl_.rows);
-  rel_.upload(pl_.rel.empty() ? std::vector<int>{0} : pl_.rel);
-  child_ptr_.upload(pl_.child_ptr);
-  child_list_.upload(pl_.child_list.empty() ? std::vector<int>{0} : pl_.child_list);
-  post_.upload(pl_.post);
+  auto upi = [](DevBuf<int>& b, const std::vector<int>& v) {
+    if (v.empty()) b.upload(std::vector<int>{0});
+    else b.upload(v);
+  };
+  upi(sn_first_, pl_.sn_first);
+  upi(rows_ptr_, pl_.rows_ptr);
+  upi(rows_, pl_.rows);
+  upi(rel_, pl_.rel);
+  upi(child_ptr_, pl_.child_ptr);
+  upi(child_list_, pl_.child_list);
+  upi(post_, pl_.post);
+  upi(sn_parent_, pl_.sn_parent);
   l_off_.upload(pl_.l_off);
   u_off_.upload(pl_.u_off);
-  a_src_.upload(pl_.a_src);
-  a_dst_.upload(pl_.a_dst);
-  std::vector<int64_t> dpos(pl_.n), uvo(pl_.ns + 1, 0);
+  upi(a_src_, pl_.a_src);
+  a_dst_.upload(pl_.a_dst.empty() ? std::vector<int64_t>{0} : pl_.a_dst);
+  std::vector<int64_t> dpos(std::max(pl_.n, 1)), uvo(pl_.ns + 1, 0), dvo(pl_.ns + 1, 0);
   for (int s = 0; s < pl_.ns; s++) {
     int f = pl_.nc(s) + pl_.nr(s);
     for (int j = pl_.sn_first[s]; j < pl_.sn_first[s + 1]; j++) {
@@ -38,15 +48,16 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
       dpos[j] = pl_.l_off[s] + lj * f + lj;
     }
     uvo[s + 1] = uvo[s] + pl_.nr(s);
+    dvo[s + 1] = dvo[s] + static_cast<int64_t>(cdiv(pl_.nc(s), NB)) * NB * NB;
   }
   diag_pos_.upload(dpos);
   uvec_off_.upload(uvo);
+  dinv_off_.upload(dvo);
   L_.alloc(static_cast<size_t>(std::max<int64_t>(pl_.l_off[pl_.ns], 1)));
   U_.alloc(static_cast<size_t>(std::max<int64_t>(pl_.u_off[pl_.ns], 1)));
   ubuf_.alloc(static_cast<size_t>(std::max<int64_t>(uvo[pl_.ns], 1)));
+  dinv_.alloc(static_cast<size_t>(std::max<int64_t>(dvo[pl_.ns], 1)));
   pivot_.alloc(1);
-  build_tasks();
-  wbuf_.alloc(static_cast<size_t>(max_level_width_) * std::max(pl_.max_front, 1));
   cd_.n = pl_.n;
   cd_.ns = pl_.ns;
   cd_.sn_first = sn_first_.p;
@@ -57,30 +68,30 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   cd_.u_off = u_off_.p;
   cd_.child_ptr = child_ptr_.p;
   cd_.child_list = child_list_.p;
+  cd_.sn_parent = sn_parent_.p;
   cd_.L = L_.p;
   cd_.U = U_.p;
   cd_.pivot = pivot_.p;
+  cd_.dinv = dinv_.p;
+  cd_.dinv_off = dinv_off_.p;
+  build_tasks();
+  build_solve();
 }
 
 void CholEngine::build_tasks()
 {
   const int nlev = static_cast<int>(pl_.level_ptr.size()) - 1;
   lv_.assign(nlev, Level{});
-  std::vector<int> sns, eap, eacol, eacptr{0}, eacc, eaccol;
-  std::vector<int> potrf, trsm_sn, trsm_tile, upd_sn, upd_tr, upd_tc, syrk_sn, syrk_tr, syrk_tc;
+  std::vector<int> eap, eacol, eacptr{0}, eacc, eaccol;
+  std::vector<int> potrf, trsm_sn, trsm_tr, trsm_tc, upd_sn, upd_tr, upd_tc;
+  std::vector<int> syrk_sn, syrk_tr, syrk_tc;
   struct Contrib {
     int t, c, r2;
   };
   std::vector<Contrib> con;
-  max_level_width_ = 1;
   for (int l = 0; l < nlev; l++) {
     Level& L = lv_[l];
     const int b0 = pl_.level_ptr[l], b1 = pl_.level_ptr[l + 1];
-    max_level_width_ = std::max(max_level_width_, b1 - b0);
-    L.sns.off = static_cast<int>(sns.size());
-    for (int q = b0; q < b1; q++) sns.push_back(pl_.level_sn[q]);
-    L.sns.cnt = b1 - b0;
-    // extend-add tasks: panel part and update-matrix part
     for (int part = 0; part < 2; part++) {
       Launch& la = part == 0 ? L.ea_panel : L.ea_u;
       la.off = static_cast<int>(eap.size());
@@ -111,7 +122,6 @@ void CholEngine::build_tasks()
       }
       la.cnt = static_cast<int>(eap.size()) - la.off;
     }
-    // blocked panel factorization steps
     int nsteps = 0;
     for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, cdiv(pl_.nc(pl_.level_sn[q]), NB));
     L.potrf.resize(nsteps);
@@ -128,12 +138,12 @@ void CholEngine::build_tasks()
         if (k0 >= nc) continue;
         const int b = std::min(NB, nc - k0);
         potrf.push_back(s);
-        const int below = f - (k0 + b);
-        for (int t = 0; t < cdiv(below, 64); t++) {
+        const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
+        for (int t = 0; t < rtiles; t++) {
           trsm_sn.push_back(s);
-          trsm_tile.push_back(t);
+          trsm_tr.push_back(t);
+          trsm_tc.push_back(0);
         }
-        const int ctiles = cdiv(nc - (k0 + b), TS), rtiles = cdiv(f - (k0 + b), TS);
         for (int tc = 0; tc < ctiles; tc++)
           for (int tr = tc; tr < rtiles; tr++) {
             upd_sn.push_back(s);
@@ -147,7 +157,7 @@ void CholEngine::build_tasks()
     }
     L.syrk.off = static_cast<int>(syrk_sn.size());
     for (int q = b0; q < b1; q++) {
-      const int s = pl_.level_sn[q], nt = cdiv(pl_.nr(s), TS);
+      const int s = pl_.level_sn[q], nt = cdiv(pl_.nr(s), NB);
       for (int tc = 0; tc < nt; tc++)
         for (int tr = tc; tr < nt; tr++) {
           syrk_sn.push_back(s);
@@ -161,7 +171,6 @@ void CholEngine::build_tasks()
     if (v.empty()) v.push_back(0);
     b.upload(v);
   };
-  up(t_sns_, sns);
   up(t_ea_p_, eap);
   up(t_ea_col_, eacol);
   up(t_ea_cptr_, eacptr);
@@ -169,13 +178,75 @@ void CholEngine::build_tasks()
   up(t_ea_ccol_, eaccol);
   up(t_potrf_, potrf);
   up(t_trsm_sn_, trsm_sn);
-  up(t_trsm_tile_, trsm_tile);
+  up(t_trsm_tr_, trsm_tr);
+  up(t_trsm_tc_, trsm_tc);
   up(t_upd_sn_, upd_sn);
   up(t_upd_tr_, upd_tr);
   up(t_upd_tc_, upd_tc);
   up(t_syrk_sn_, syrk_sn);
   up(t_syrk_tr_, syrk_tr);
   up(t_syrk_tc_, syrk_tc);
+}
+
+void CholEngine::build_solve()
+{
+  const int ns = pl_.ns;
+  std::vector<int> fwd, bwd, pblk(ns + 1, 0);
+  for (int s = 0; s < ns; s++) pblk[s + 1] = pblk[s] + cdiv(pl_.nc(s), NB);
+  for (int s = 0; s < ns; s++) {   // postorder: children first
+    for (int b = 0; b < cdiv(pl_.nc(s), NB); b++) fwd.insert(fwd.end(), {s, 0, b});
+    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++) fwd.insert(fwd.end(), {s, 1, u});
+  }
+  for (int s = ns - 1; s >= 0; s--)   // parents first
+    for (int b = cdiv(pl_.nc(s), NB) - 1; b >= 0; b--) bwd.insert(bwd.end(), {s, b});
+  // pull lists: front row of a parent <- entries of its children's update vectors
+  std::vector<int64_t> foff(ns + 1, 0);
+  for (int s = 0; s < ns; s++) foff[s + 1] = foff[s] + pl_.nc(s) + pl_.nr(s);
+  std::vector<int> cnt(static_cast<size_t>(foff[ns]) + 1, 0);
+  int64_t uo = 0;
+  std::vector<int64_t> uvo(ns + 1, 0);
+  for (int s = 0; s < ns; s++) uvo[s + 1] = uvo[s] + pl_.nr(s);
+  (void)uo;
+  for (int c = 0; c < ns; c++) {
+    int p = pl_.sn_parent[c];
+    if (p < 0) continue;
+    for (int k = pl_.rows_ptr[c]; k < pl_.rows_ptr[c + 1]; k++) cnt[foff[p] + pl_.rel[k] + 1]++;
+  }
+  for (size_t i = 0; i + 1 < cnt.size(); i++) cnt[i + 1] += cnt[i];
+  std::vector<int> src(cnt.back() > 0 ? cnt.back() : 1), fill(cnt.begin(), cnt.end() - 1);
+  for (int p = 0; p < ns; p++)
+    for (int t = pl_.child_ptr[p]; t < pl_.child_ptr[p + 1]; t++) {
+      const int c = pl_.child_list[t];
+      for (int k = pl_.rows_ptr[c]; k < pl_.rows_ptr[c + 1]; k++)
+        src[fill[foff[p] + pl_.rel[k]]++] =
+            static_cast<int>(uvo[c] + (k - pl_.rows_ptr[c]));
+    }
+  auto upi = [](DevBuf<int>& b, std::vector<int>& v) {
+    if (v.empty()) v.push_back(0);
+    b.upload(v);
+  };
+  upi(fwd_items_, fwd);
+  upi(bwd_items_, bwd);
+  upi(pblk_off_, pblk);
+  upi(pull_ptr_, cnt);
+  upi(pull_src_, src);
+  front_off_.upload(foff);
+  const int nflags = 2 * pblk[ns] + 2 * ns + 1;
+  flags_.alloc(nflags);
+  sd_.fwd_items = fwd_items_.p;
+  sd_.n_fwd = static_cast<int>(fwd.size() / 3);
+  sd_.bwd_items = bwd_items_.p;
+  sd_.n_bwd = static_cast<int>(bwd.size() / 2);
+  sd_.pblk_off = pblk_off_.p;
+  sd_.sn_parent = sn_parent_.p;
+  sd_.uvec_off = uvec_off_.p;
+  sd_.pull_ptr = pull_ptr_.p;
+  sd_.pull_src = pull_src_.p;
+  sd_.front_off = front_off_.p;
+  sd_.ubuf = ubuf_.p;
+  sd_.flags = flags_.p;
+  sd_.n_pblk = pblk[ns];
+  dev::fill_i(flags_.p, 0, nflags, st_);
 }
 
 void CholEngine::zero_L() { dev::fill(L_.p, 0.0, static_cast<int64_t>(L_.n), st_); }
@@ -199,14 +270,15 @@ void CholEngine::factor(double pivot_tol)
     dev::extend_add(cd_, L.ea_panel.cnt, t_ea_p_.p + L.ea_panel.off, t_ea_col_.p + L.ea_panel.off,
                     t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
     for (size_t j = 0; j < L.potrf.size(); j++) {
-      dev::potrf_block(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, static_cast<int>(j), st_);
-      dev::trsm_block(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
-                      t_trsm_tile_.p + L.trsm[j].off, static_cast<int>(j), st_);
-      dev::panel_update(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,
-                        t_upd_tc_.p + L.upd[j].off, static_cast<int>(j), st_);
+      const int step = static_cast<int>(j);
+      dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, st_);
+      dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off, t_trsm_tr_.p + L.trsm[j].off,
+                    t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
+      dev::dense_nt(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,
+                    t_upd_tc_.p + L.upd[j].off, 0, step, st_);
     }
-    dev::syrk_update(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
-                     t_syrk_tc_.p + L.syrk.off, st_);
+    dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
+                  t_syrk_tc_.p + L.syrk.off, 1, 0, st_);
     dev::extend_add(cd_, L.ea_u.cnt, t_ea_p_.p + L.ea_u.off, t_ea_col_.p + L.ea_u.off,
                     t_ea_cptr_.p + L.ea_u.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
   }
@@ -220,15 +292,20 @@ int CholEngine::pivot()
   return h == INT_MAX ? -1 : h;
 }
 
+int CholEngine::solve_error()
+{
+  int h = 0;
+  const int idx = 2 * sd_.n_pblk + 2 * pl_.ns;
+  HIPCHK(hipMemcpyAsync(&h, flags_.p + idx, sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return h;
+}
+
 void CholEngine::solve(double* x)
 {
-  const int nlev = static_cast<int>(lv_.size());
-  for (int l = 0; l < nlev; l++)
-    dev::trsv_forward(cd_, lv_[l].sns.cnt, t_sns_.p + lv_[l].sns.off, x, ubuf_.p, uvec_off_.p,
-                      wbuf_.p, pl_.max_front, st_);
-  for (int l = nlev - 1; l >= 0; l--)
-    dev::trsv_backward(cd_, lv_[l].sns.cnt, t_sns_.p + lv_[l].sns.off, x, wbuf_.p, pl_.max_front,
-                       st_);
+  dev::solve_reset(sd_, pl_.ns, st_);
+  dev::solve_forward(cd_, sd_, x, st_);
+  dev::solve_backward(cd_, sd_, x, st_);
 }
 
 }  // namespace hykkt

@@ -66,6 +66,9 @@ class CholEngine {
   void factor(double pivot_tol);
   // First failing pivot (internal index) or -1; synchronizes the stream.
   int pivot();
+  // Nonzero if a triangular-solve sweep hit its wait bound since the last
+  // call (synchronizes the stream, clears the flag).
+  int solve_error();
   // Solve L L^T x = b in place, x in internal (postordered) order.
   void solve(double* x);
   // Device arrays of the assembly map (a's lower-triangle entries).
@@ -79,25 +82,28 @@ class CholEngine {
     int off = 0, cnt = 0;
   };
   struct Level {
-    Launch sns, ea_panel, ea_u, syrk;
+    Launch ea_panel, ea_u, syrk;
     std::vector<Launch> potrf, trsm, upd;
   };
   void build_tasks();
+  void build_solve();
 
   CholPlan pl_;
   hipStream_t st_;
   dev::CholDev cd_{};
-  DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_;
-  DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_;
+  dev::SolveDev sd_{};
+  DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_, sn_parent_;
+  DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_, dinv_off_, front_off_;
   DevBuf<int> a_src_;
-  DevBuf<double> L_, U_, ubuf_, wbuf_;
+  DevBuf<double> L_, U_, ubuf_, dinv_;
   DevBuf<int> pivot_;
   // task lists
   std::vector<Level> lv_;
-  DevBuf<int> t_sns_, t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
-  DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tile_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
+  DevBuf<int> t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
+  DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tr_, t_trsm_tc_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
   DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
-  int max_level_width_ = 0;
+  // solve items
+  DevBuf<int> fwd_items_, bwd_items_, pblk_off_, pull_ptr_, pull_src_, flags_;
 };
 
 // The Schur-complement operator S = Jh A^{-1} Jh^T with A factored by `chol`

@@ -100,35 +100,49 @@ struct CholDev {
   const int64_t* u_off;     // ns+1
   const int* child_ptr;
   const int* child_list;
+  const int* sn_parent;
   double* L;
   double* U;
   int* pivot;               // device int: first failing pivot (min), INT_MAX if none
   double pivot_tol;
+  double* dinv;             // inverted 128x128 diagonal blocks (column-major)
+  const int64_t* dinv_off;  // ns+1, offset of supernode s's first block in dinv
 };
 
 // Extend-add tasks: task t = (parent p, parent front column col);
 // contributions [cptr[t], cptr[t+1]) are (child, child-local column) pairs.
 void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task_col,
                 const int* cptr, const int* contrib_child, const int* contrib_col, hipStream_t st);
-// Block step `step` of the panel factorization for the supernodes in `sns`.
-void potrf_block(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st);
-// TRSM tasks (supernode, 64-row tile index below the diagonal block of `step`).
-void trsm_block(const CholDev& c, int ntasks, const int* task_sn, const int* task_tile,
-                int step, hipStream_t st);
-// Trailing panel update tasks (supernode, tile row, tile col), K = block `step`.
-void panel_update(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
-                  const int* task_tc, int step, hipStream_t st);
-// U = -L21 L21^T tasks (supernode, tile row, tile col) over the nr x nr lower part.
-void syrk_update(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
-                 const int* task_tc, hipStream_t st);
+// Factor + invert the 128x128 diagonal block of step `step` of each supernode.
+void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st);
+// MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
+// panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the
+// rows below block `step` with its inverted diagonal block.
+void dense_nt(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
+              const int* task_tc, int mode, int step, hipStream_t st);
 
-// Level-scheduled supernodal solves of L L^T x = b in place on x (internal
-// order).  ubuf holds the forward update vectors (sum of nr doubles).
-// wbuf: scratch of nsn * max_front doubles.
-void trsv_forward(const CholDev& c, int nsn, const int* sns, double* x, double* ubuf,
-                  const int64_t* uvec_off, double* wbuf, int max_front, hipStream_t st);
-void trsv_backward(const CholDev& c, int nsn, const int* sns, double* x, double* wbuf,
-                   int max_front, hipStream_t st);
+// Single-launch supernodal solves over the whole tree (L y = b, then L^T x = y,
+// in place on x, internal order).  Work items are 128-row blocks of the
+// fronts, dispatched in a dependency-respecting order; each item waits for the
+// items it depends on through flags in `sync` (reset by solve_reset).
+struct SolveDev {
+  const int* fwd_items;     // 3 ints per item: supernode, kind (0 pivot, 1 update), block
+  int n_fwd;
+  const int* bwd_items;     // 2 ints per item: supernode, block
+  int n_bwd;
+  const int* pblk_off;      // ns+1: global index of the first pivot block of s
+  const int* sn_parent;
+  const int64_t* uvec_off;  // ns+1 offsets of the forward update vectors
+  const int* pull_ptr;      // front-row pull lists (children contributions)
+  const int* pull_src;
+  const int64_t* front_off; // ns+1: offset of s's front rows in the pull_ptr index
+  double* ubuf;
+  int* flags;               // n_pblk pivot-block flags, then ns fwd counters, ns bwd counters, err
+  int n_pblk;
+};
+void solve_reset(const SolveDev& sd, int ns, hipStream_t st);
+void solve_forward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st);
+void solve_backward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st);
 
 }  // namespace dev
 }  // namespace hykkt

@@ -517,6 +517,8 @@ int Solver::execute()
   HIPCHK(hipMemcpyAsync(hr, r, 8 * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   hpiv = ch.pivot();
+  HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
+                "triangular-solve sweep exceeded its wait bound (results invalid)");
   double nb = hr[4] + hr[5] + hr[6] + hr[7];
   stats.resid2 = std::sqrt(hr[0] + hr[1]) / std::sqrt(nb);
   stats.resid4 = std::sqrt(hr[0] + hr[1] + hr[2] + hr[3]) / std::sqrt(nb);

@@ -165,6 +165,14 @@ int hykkt_set_permutation(hykkt_solver s, const int* perm, int n);
  * Returns the number of elements copied or a negative error. */
 long long hykkt_debug_copy(hykkt_solver s, int what, void* host_out, long long capacity);
 
+/* Device memory helpers for callers without a HIP runtime of their own (for
+ * example a ctypes binding), so that inputs can be made resident in HBM. */
+int hykkt_device_malloc(void** ptr, unsigned long long bytes);
+int hykkt_device_free(void* ptr);
+int hykkt_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
+int hykkt_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
+int hykkt_device_synchronize(void);
+
 /* ---------------------------------------------------------------------------
  * Component entry points mirroring the reference's step classes, used by
  * their known-answer tests.  Host arrays in and out.

@@ -57,6 +57,7 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   U_.alloc(static_cast<size_t>(std::max<int64_t>(pl_.u_off[pl_.ns], 1)));
   ubuf_.alloc(static_cast<size_t>(std::max<int64_t>(uvo[pl_.ns], 1)));
   dinv_.alloc(static_cast<size_t>(std::max<int64_t>(dvo[pl_.ns], 1)));
+  dev::fill(dinv_.p, 0.0, static_cast<int64_t>(dinv_.n), st_);
   pivot_.alloc(1);
   cd_.n = pl_.n;
   cd_.ns = pl_.ns;
@@ -138,6 +139,7 @@ void CholEngine::build_tasks()
         if (k0 >= nc) continue;
         const int b = std::min(NB, nc - k0);
         potrf.push_back(s);
+        L.potrf[j].maxb = std::max(L.potrf[j].maxb, b);
         const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
         for (int t = 0; t < rtiles; t++) {
           trsm_sn.push_back(s);
@@ -271,7 +273,7 @@ void CholEngine::factor(double pivot_tol)
                     t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
-      dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, st_);
+      dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb, st_);
       dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off, t_trsm_tr_.p + L.trsm[j].off,
                     t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
       dev::dense_nt(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,

@@ -79,7 +79,7 @@ class CholEngine {
 
  private:
   struct Launch {
-    int off = 0, cnt = 0;
+    int off = 0, cnt = 0, maxb = 0;
   };
   struct Level {
     Launch ea_panel, ea_u, syrk;

@@ -113,8 +113,10 @@ struct CholDev {
 // contributions [cptr[t], cptr[t+1]) are (child, child-local column) pairs.
 void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task_col,
                 const int* cptr, const int* contrib_child, const int* contrib_col, hipStream_t st);
-// Factor + invert the 128x128 diagonal block of step `step` of each supernode.
-void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st);
+// Factor + invert the (<= 128 wide) diagonal block of step `step` of each
+// supernode; max_b = largest block in the launch (selects the LDS footprint).
+// dinv slots must be zero-initialized once (only the b x b lower part is written).
+void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st);
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the
 // rows below block `step` with its inverted diagonal block.

@@ -183,7 +183,21 @@ __global__ void __launch_bounds__(256) k_dense_nt(CholDev c, const int* __restri
 // Factor the diagonal block of step `step` (b <= 128) in LDS, write L_kk back
 // and its inverse (lower triangular, zero-padded to 128 x 128, column-major)
 // to dinv.  LDS holds one 128 x 129 array P: L(r, c) (r >= c) at P[r][c]; the
-// inverse X(i, j) (i > j) at P[j][i] and X(j, j) at P[j][128].
+// inverse X(i, j) (i > j) at P[j][i] and X(j, j) at P[j][128]; plus a 3 x 32 x 33
+// scratch for the off-diagonal blocks of the inverse.  The block is padded to
+// 128 with the identity.
+//
+// Both steps are blocked by 32.  Cholesky: wave 0 factors the 32x32 diagonal
+// block in registers (lane r holds row r; columns move between lanes by
+// shuffles), then every thread solves one row below against it, then the
+// trailing lower triangle is updated.  Inverse: wave w inverts diagonal block
+// w in registers (lane j computes column j), then the block diagonals below
+// follow from X(i,j) = -X(i,i) * sum_{k=j}^{i-1} L(i,k) X(k,j).
+constexpr int SUB = 32;
+
+// BK: padded block edge (32, 64 or 128) chosen per launch from the largest
+// diagonal block in it, so small supernodes need little LDS.
+template <int BK>
 __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
                                                    int step)
 {
@@ -192,69 +206,221 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   const SnInfo si = sninfo(c, s);
   const int k0 = step * BT;
   if (k0 >= si.nc) return;
-  const int b = min(BT, si.nc - k0);
-  double (*P)[BT + 1] = reinterpret_cast<double (*)[BT + 1]>(smem);
+  const int b = min(BK, si.nc - k0);
+  double (*P)[BK + 1] = reinterpret_cast<double (*)[BK + 1]>(smem);
+  double (*T)[SUB][SUB + 1] = reinterpret_cast<double (*)[SUB][SUB + 1]>(smem + BK * (BK + 1));
   __shared__ int bad;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
-  for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
-    int col = idx / b, r = idx % b;
-    if (r >= col) P[r][col] = D[r + static_cast<int64_t>(col) * si.f];
-  }
-  if (threadIdx.x == 0) bad = INT_MAX;
+  for (int col = tid / 64; col < BK; col += 4)
+    for (int r = (tid & 63) + col; r < BK; r += 64)
+      P[r][col] = (r < b && col < b) ? D[r + static_cast<int64_t>(col) * si.f]
+                                     : (r == col ? 1.0 : 0.0);
+  if (tid == 0) bad = INT_MAX;
   __syncthreads();
-  // right-looking unblocked Cholesky of the b x b block
-  for (int k = 0; k < b; k++) {
-    if (threadIdx.x == 0) {
-      double d = P[k][k];
-      if (!(d > 0.0) || !(sqrt(d) > c.pivot_tol)) {
-        if (bad == INT_MAX) bad = si.a + k0 + k;
-        d = 1.0;   // keep the arithmetic finite; the failure is reported
+
+  // ---- blocked Cholesky ----
+  for (int kb = 0; kb < BK; kb += SUB) {
+    if (wave == 0) {
+      // lane r holds row r; entries above the diagonal are updated too (they
+      // are never stored), which keeps the loop free of lane-dependent branches
+      const int r = lane & (SUB - 1);
+      double v[SUB];
+#pragma unroll
+      for (int q = 0; q < SUB; q++) v[q] = (q <= r) ? P[kb + r][kb + q] : 0.0;
+      int badk = -1;
+#pragma unroll
+      for (int k = 0; k < SUB; k++) {
+        double dkk = __shfl(v[k], k, 64);
+        const bool ok = (dkk > 0.0) && (sqrt(dkk) > c.pivot_tol);
+        badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
+        const double d = sqrt(ok ? dkk : 1.0);   // failure reported, arithmetic kept finite
+        v[k] = (r == k) ? d : v[k] / d;
+        const double vk = v[k];
+#pragma unroll
+        for (int q = k + 1; q < SUB; q++) v[q] -= vk * __shfl(vk, q, 64);
       }
-      P[k][k] = sqrt(d);
+      if (lane == 0 && badk >= 0 && bad == INT_MAX) bad = si.a + k0 + kb + badk;
+      if (lane < SUB) {
+#pragma unroll
+        for (int q = 0; q < SUB; q++)
+          if (q <= r) P[kb + r][kb + q] = v[q];
+      }
     }
     __syncthreads();
-    const double dk = P[k][k];
-    for (int r = k + 1 + threadIdx.x; r < b; r += blockDim.x) P[r][k] /= dk;
-    __syncthreads();
-    const int m = b - k - 1;
-    for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-      int col = k + 1 + idx / m, r = k + 1 + idx % m;
-      if (col <= r) P[r][col] -= P[r][k] * P[col][k];
+    const int rbeg = kb + SUB;
+    if (rbeg < BK) {
+      // rows below: x = a * L_kb^{-T}, one row per thread, row held in registers
+      for (int r = rbeg + tid; r < BK; r += blockDim.x) {
+        double x[SUB];
+#pragma unroll
+        for (int q = 0; q < SUB; q++) x[q] = P[r][kb + q];
+#pragma unroll
+        for (int k = 0; k < SUB; k++) {
+          double t0 = x[k], t1 = 0.0;
+#pragma unroll
+          for (int q = 0; q < k; q++) {
+            if (q & 1) t1 -= x[q] * P[kb + k][kb + q];
+            else t0 -= x[q] * P[kb + k][kb + q];
+          }
+          x[k] = (t0 + t1) / P[kb + k][kb + k];
+        }
+#pragma unroll
+        for (int q = 0; q < SUB; q++) P[r][kb + q] = x[q];
+      }
+      __syncthreads();
+      // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k], r >= cc >= rbeg,
+      // in 4x4 register tiles (lower tiles only)
+      const int mt = (BK - rbeg) / 4;
+      for (int e = tid; e < mt * mt; e += blockDim.x) {
+        const int ti = e / mt, tj = e % mt;
+        if (tj > ti) continue;
+        const int r0 = rbeg + 4 * ti, c0 = rbeg + 4 * tj;
+        double acc[4][4] = {};
+#pragma unroll 4
+        for (int k = kb; k < kb + SUB; k++) {
+          double a[4], bb[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            a[q] = P[r0 + q][k];
+            bb[q] = P[c0 + q][k];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int w = 0; w < 4; w++) acc[q][w] += a[q] * bb[w];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int w = 0; w < 4; w++)
+            if (c0 + w <= r0 + q) P[r0 + q][c0 + w] -= acc[q][w];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  // X = L^{-1} row by row: X(i, j) = -(sum_{k<i} L(i,k) X(k,j)) / L(i,i), j < i,
-  // thread j owns column j; L(i, k) is read by all threads at once (broadcast).
-  const int j = threadIdx.x;
-  if (j < b) P[j][BT] = 1.0 / P[j][j];
+
+  // ---- inverse of L ----
+  {
+    // wave w inverts diagonal block w; lane j < 32 computes column j
+    const int base = wave * SUB, j = lane;
+    if (j < SUB && base < BK) {
+      double x[SUB];
+#pragma unroll
+      for (int i = 0; i < SUB; i++) {
+        double t0 = (i == j) ? 1.0 : 0.0, t1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; k++) {
+          if (k & 1) t1 -= P[base + i][base + k] * x[k];
+          else t0 -= P[base + i][base + k] * x[k];
+        }
+        x[i] = (i >= j) ? (t0 + t1) / P[base + i][base + i] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < SUB; i++) {
+        if (i == j) P[base + j][BK] = x[i];
+        else if (i > j) P[base + j][base + i] = x[i];
+      }
+    }
+  }
   __syncthreads();
-  for (int i = 1; i < b; i++) {
-    if (j < i) {
-      double t = P[i][j] * P[j][BT];
-      for (int k = j + 1; k < i; k++) t += P[i][k] * P[j][k];
-      P[j][i] = -t / P[i][i];
+  auto X = [&](int i, int j) -> double { return (i == j) ? P[j][BK] : P[j][i]; };
+  // block diagonals: T_q = sum_{k=j..ib-1} L(i,k) X(k,j), then X(i,j) = -X_ii T_q,
+  // 4x4 register tiles, (q, tile) spread over the threads
+  for (int d = 1; d < BK / SUB; d++) {
+    const int nblk = BK / SUB - d;
+    constexpr int TPB4 = (SUB / 4) * (SUB / 4);   // 4x4 tiles per 32x32 block
+    for (int e = tid; e < nblk * TPB4; e += blockDim.x) {
+      const int q = e / TPB4, ti = (e % TPB4) / (SUB / 4), tj = e % (SUB / 4);
+      const int ib = (q + d) * SUB, i0 = ib + 4 * ti, j0 = q * SUB + 4 * tj;
+      double acc[4][4] = {};
+      // X(k, j) = 0 for k < j: start at the tile's first column; the first
+      // four k straddle the diagonal of X, beyond them X(k, j) = P[j][k]
+#pragma unroll
+      for (int k = j0; k < j0 + 4; k++) {
+        double a[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          a[u] = P[i0 + u][k];
+          const int j = j0 + u;
+          xv[u] = (k > j) ? P[j][k] : ((k == j) ? P[j][BK] : 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int w = 0; w < 4; w++) acc[u][w] += a[u] * xv[w];
+      }
+      for (int k = j0 + 4; k < ib; k++) {
+        double a[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          a[u] = P[i0 + u][k];
+          xv[u] = P[j0 + u][k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int w = 0; w < 4; w++) acc[u][w] += a[u] * xv[w];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int w = 0; w < 4; w++) T[q][4 * ti + u][4 * tj + w] = acc[u][w];
+    }
+    __syncthreads();
+    for (int e = tid; e < nblk * TPB4; e += blockDim.x) {
+      const int q = e / TPB4, ti = (e % TPB4) / (SUB / 4), tj = e % (SUB / 4);
+      const int ib = (q + d) * SUB, i0 = ib + 4 * ti, j0 = q * SUB + 4 * tj;
+      double acc[4][4] = {};
+      // X(i, ib + k) for k below the tile's rows is P[ib + k][i]; the last four
+      // k straddle the diagonal of X
+      for (int k = 0; k < 4 * ti; k++) {
+        double xi[4], tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          xi[u] = P[ib + k][i0 + u];
+          tv[u] = T[q][k][4 * tj + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int w = 0; w < 4; w++) acc[u][w] += xi[u] * tv[w];
+      }
+#pragma unroll
+      for (int k = 4 * ti; k < 4 * ti + 4; k++) {
+        double xi[4], tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int i = i0 + u, kk = ib + k;
+          xi[u] = (i > kk) ? P[kk][i] : ((i == kk) ? P[kk][BK] : 0.0);
+          tv[u] = T[q][k][4 * tj + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int w = 0; w < 4; w++) acc[u][w] += xi[u] * tv[w];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int w = 0; w < 4; w++) P[j0 + w][i0 + u] = -acc[u][w];
     }
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
-    int col = idx / b, r = idx % b;
-    if (r >= col) D[r + static_cast<int64_t>(col) * si.f] = P[r][col];
-  }
+
+  for (int col = tid / 64; col < b; col += 4)
+    for (int r = (tid & 63) + col; r < b; r += 64) D[r + static_cast<int64_t>(col) * si.f] = P[r][col];
+  // inverse into the 128 x 128 dinv slot (column-major); entries outside the
+  // b x b lower triangle stay at the zeros written when the slot was created
   double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
-  for (int idx = threadIdx.x; idx < BT * BT; idx += blockDim.x) {
-    int col = idx / BT, r = idx % BT;
-    double x = 0.0;
-    if (r < b && col < b && r >= col) x = (r == col) ? P[col][BT] : P[col][r];
-    W[idx] = x;
-  }
-  if (threadIdx.x == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
+  for (int col = tid / 64; col < b; col += 4)
+    for (int r = (tid & 63) + col; r < b; r += 64) W[static_cast<int64_t>(col) * BT + r] = X(r, col);
+  if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
 }
 
 }  // namespace
 
 size_t dense_nt_smem() { return 2 * KC * LDA_S * sizeof(double); }
-size_t potrf_inv_smem() { return BT * (BT + 1) * sizeof(double); }
-
 void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, const int* ttc,
               int mode, int step, hipStream_t st)
 {
@@ -263,18 +429,32 @@ void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, cons
   HIPCHK(hipGetLastError());
 }
 
-void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st)
+template <int BK>
+static size_t potrf_smem()
 {
-  if (nsn <= 0) return;
+  return (BK * (BK + 1) + (BK / SUB - 1) * SUB * (SUB + 1)) * sizeof(double);
+}
+
+template <int BK>
+static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st)
+{
   static bool attr = false;
   if (!attr) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_potrf_inv),
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_potrf_inv<BK>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(potrf_inv_smem())));
+                               static_cast<int>(potrf_smem<BK>())));
     attr = true;
   }
-  k_potrf_inv<<<nsn, 256, potrf_inv_smem(), st>>>(c, sns, step);
+  k_potrf_inv<BK><<<nsn, 256, potrf_smem<BK>(), st>>>(c, sns, step);
   HIPCHK(hipGetLastError());
+}
+
+void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st)
+{
+  if (nsn <= 0) return;
+  if (max_b <= 32) launch_potrf<32>(c, nsn, sns, step, st);
+  else if (max_b <= 64) launch_potrf<64>(c, nsn, sns, step, st);
+  else launch_potrf<128>(c, nsn, sns, step, st);
 }
 
 }  // namespace dev

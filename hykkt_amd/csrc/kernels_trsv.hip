@@ -128,13 +128,18 @@ __global__ void __launch_bounds__(256) k_solve_fwd(CholDev c, SolveDev sd, doubl
   for (int j = 0; j < jmax; j++) {
     wait_geq(fflag + sd.pblk_off[s] + j, 1, err);
     const int bj = min(SB, si.nc - j * SB);
-    if (threadIdx.x < bj) yv[threadIdx.x] = x[si.a + j * SB + threadIdx.x];
+    if (threadIdx.x < SB) yv[threadIdx.x] = (threadIdx.x < bj) ? x[si.a + j * SB + threadIdx.x] : 0.0;
     __syncthreads();
     double tp = 0.0;
     if (row < nrows) {
-      const double* col = si.Ls + static_cast<int64_t>(j * SB) * si.f + r;
-      const int cb = half * (SB / 2), ce = min(bj, cb + SB / 2);
-      for (int cc = cb; cc < ce; cc++) tp += col[static_cast<int64_t>(cc) * si.f] * yv[cc];
+      // 64 independent loads per thread (one column of L each), then the FMAs
+      const double* col = si.Ls + static_cast<int64_t>(j * SB + half * 64) * si.f + r;
+      const int cn = min(64, bj - half * 64);
+      double lv[64];
+#pragma unroll
+      for (int q = 0; q < 64; q++) lv[q] = (q < cn) ? col[static_cast<int64_t>(q) * si.f] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 64; q++) tp += lv[q] * yv[half * 64 + q];
     }
     if (half == 1) part[row] = tp;
     __syncthreads();
@@ -146,11 +151,24 @@ __global__ void __launch_bounds__(256) k_solve_fwd(CholDev c, SolveDev sd, doubl
     if (half == 0) part[row] = (row < nrows) ? v : 0.0;
     __syncthreads();
     const double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB;
-    if (half == 0 && row < nrows) {
-      double y = 0.0;
-      for (int cc = 0; cc <= row; cc++) y += W[cc * SB + row] * part[cc];
-      x[si.a + r] = y;
+    // y(row) = sum_{cc <= row} W(row, cc) v(cc): the two halves of the
+    // threads take the columns [0,64) and [64,128); loads issued 32 at a time
+    double y = 0.0;
+    if (row < nrows) {
+      const double* wr = W + row + static_cast<int64_t>(half * 64) * SB;
+#pragma unroll
+      for (int q0 = 0; q0 < 64; q0 += 32) {
+        double w[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) w[q] = wr[static_cast<int64_t>(q0 + q) * SB];
+#pragma unroll
+        for (int q = 0; q < 32; q++) y += w[q] * part[half * 64 + q0 + q];
+      }
     }
+    __syncthreads();
+    if (half == 1) yv[row] = y;
+    __syncthreads();
+    if (half == 0 && row < nrows) x[si.a + r] = y + yv[row];
     signal_add(fflag + sd.pblk_off[s] + blk);
   } else {
     if (half == 0 && row < nrows) sd.ubuf[sd.uvec_off[s] + (r - si.nc)] = v;
@@ -159,9 +177,36 @@ __global__ void __launch_bounds__(256) k_solve_fwd(CholDev c, SolveDev sd, doubl
 }
 
 // ---------------------------------------------------------------- backward
+// Partial of sum_{k < nrow} L(row0 + k, c0 + cc) * xs[k] for this thread's
+// column cc = tid % 128 over its half of the rows.  Lp points at (row0, c0)
+// of a column-major panel with leading dimension f.  Each thread walks a
+// contiguous stretch of its column, 16 independent loads at a time (one
+// 128-byte line), so many lines are in flight per workgroup.
+__device__ __forceinline__ double gemv_t_cols(const double* Lp, int64_t f, int nrow, int ncol,
+                                              const double* xs)
+{
+  const int cc = threadIdx.x & (SB - 1), half = threadIdx.x >> 7;
+  if (cc >= ncol) return 0.0;
+  const int h = ((nrow + 31) / 32) * 16;          // split on a 16-row boundary
+  const int rb = half ? min(h, nrow) : 0, re = half ? nrow : min(h, nrow);
+  const double* col = Lp + static_cast<int64_t>(cc) * f;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int k = rb;
+  for (; k + 16 <= re; k += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = col[k + q];
+#pragma unroll
+    for (int q = 0; q < 16; q++) a[q & 3] += v[q] * xs[k + q];
+  }
+  for (; k < re; k++) a[0] += col[k] * xs[k];
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
 __global__ void __launch_bounds__(256) k_solve_bwd(CholDev c, SolveDev sd, double* __restrict__ x)
 {
   __shared__ double t[SB];
+  __shared__ double part[SB];
   __shared__ double xs[1024];
   const int s = sd.bwd_items[2 * blockIdx.x];
   const int blk = sd.bwd_items[2 * blockIdx.x + 1];
@@ -172,6 +217,7 @@ __global__ void __launch_bounds__(256) k_solve_bwd(CholDev c, SolveDev sd, doubl
   const int c0 = blk * SB;
   const int nb = min(SB, si.nc - c0);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cc = threadIdx.x & (SB - 1), half = threadIdx.x >> 7;
   const int* rows = c.rows + c.rows_ptr[s];
 
   const int p = c.sn_parent[s];
@@ -179,45 +225,49 @@ __global__ void __launch_bounds__(256) k_solve_bwd(CholDev c, SolveDev sd, doubl
     const int np_p = (c.sn_first[p + 1] - c.sn_first[p] + SB - 1) / SB;
     wait_geq(bcount + p, np_p, err);
   }
-  if (threadIdx.x < nb) t[threadIdx.x] = x[si.a + c0 + threadIdx.x];
-  __syncthreads();
-  // t -= L21(:, block)^T x(rows), x(rows) staged through LDS in chunks
+  if (threadIdx.x < SB) t[threadIdx.x] = (threadIdx.x < nb) ? x[si.a + c0 + threadIdx.x] : 0.0;
+  double acc = 0.0;
+  // L21(:, block)^T x(rows), x(rows) gathered through LDS 1024 at a time
   for (int k0 = 0; k0 < si.nr; k0 += 1024) {
     const int kn = min(1024, si.nr - k0);
+    __syncthreads();
     for (int k = threadIdx.x; k < kn; k += blockDim.x) xs[k] = x[rows[k0 + k]];
     __syncthreads();
-    for (int cc = wave; cc < nb; cc += 4) {
-      const double* col = si.Ls + static_cast<int64_t>(c0 + cc) * si.f + si.nc + k0;
-      double acc = 0.0;
-      for (int k = lane; k < kn; k += 64) acc += col[k] * xs[k];
-      acc = wave_sum(acc);
-      if (lane == 0) t[cc] -= acc;
-    }
-    __syncthreads();
+    acc += gemv_t_cols(si.Ls + static_cast<int64_t>(c0) * si.f + si.nc + k0, si.f, kn, nb, xs);
   }
-  // t -= L(B_j, block)^T x_j for the later pivot blocks of s
+  // L(B_j, block)^T x_j for the later pivot blocks of s
   for (int j = si.np - 1; j > blk; j--) {
     wait_geq(bflag + sd.pblk_off[s] + j, 1, err);
     const int bj = min(SB, si.nc - j * SB);
     for (int k = threadIdx.x; k < bj; k += blockDim.x) xs[k] = x[si.a + j * SB + k];
     __syncthreads();
-    for (int cc = wave; cc < nb; cc += 4) {
-      const double* col = si.Ls + static_cast<int64_t>(c0 + cc) * si.f + j * SB;
-      double acc = 0.0;
-      for (int k = lane; k < bj; k += 64) acc += col[k] * xs[k];
-      acc = wave_sum(acc);
-      if (lane == 0) t[cc] -= acc;
-    }
+    acc += gemv_t_cols(si.Ls + static_cast<int64_t>(c0) * si.f + j * SB, si.f, bj, nb, xs);
     __syncthreads();
   }
-  // x_block = Dinv^T t
+  if (half == 1) part[cc] = acc;
+  __syncthreads();
+  if (half == 0) t[cc] -= acc + part[cc];
+  __syncthreads();
+  // x_block = Dinv^T t: column q of W (contiguous) dotted with t, two threads
+  // per column, 16 independent loads at a time
   const double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB;
-  for (int cc = wave; cc < nb; cc += 4) {
-    double acc = 0.0;
-    for (int rr = lane; rr < nb; rr += 64)
-      if (rr >= cc) acc += W[cc * SB + rr] * t[rr];
-    acc = wave_sum(acc);
-    if (lane == 0) x[si.a + c0 + cc] = acc;
+  {
+    const double* wc = W + static_cast<int64_t>(cc) * SB + half * 64;
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int q0 = 0; q0 < 64; q0 += 16) {
+      double w[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) w[q] = wc[q0 + q];
+#pragma unroll
+      for (int q = 0; q < 16; q += 2) {
+        a0 += w[q] * t[half * 64 + q0 + q];
+        a1 += w[q + 1] * t[half * 64 + q0 + q + 1];
+      }
+    }
+    if (half == 1) part[cc] = a0 + a1;
+    __syncthreads();
+    if (half == 0 && cc < nb) x[si.a + c0 + cc] = a0 + a1 + part[cc];
   }
   signal_add(bflag + sd.pblk_off[s] + blk, bcount + s);
 }

@@ -29,6 +29,75 @@ def metric_name():
         return json.load(f)["metric"]
 
 
+# MI355X published peaks: FP64 (vector and MFMA) and HBM3E.
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# Kernels whose bound is the FP64 pipes; the rest are priced against HBM.
+COMPUTE_BOUND = {"k_dense_nt", "k_potrf_inv"}
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled against a copy kernel of
+    known size), or None if the summary has no line for it."""
+    path = os.path.join(ROOT, "profiles", "grid10k_seq20_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(f"hbm_bytes_per_launch {kernel}:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(kt, steps):
+    """Roofline entry for the kernel with the largest share of the timed
+    region: algorithmic work per launch over its mean event-timed duration."""
+    if not kt:
+        return None
+    name, v = max(kt.items(), key=lambda kv: kv[1]["ms"])
+    launches = max(v["launches"], 1)
+    sec = 1e-3 * v["ms"] / launches
+    if name in COMPUTE_BOUND:
+        achieved, peak, unit, bound = v["flops"] / launches / sec / 1e12, PEAK_FP64_TFLOPS, \
+            "TFLOP/s", "mfma"
+    else:
+        achieved, peak, unit, bound = v["bytes"] / launches / sec / 1e9, PEAK_HBM_GBS, "GB/s", \
+            "hbm"
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": traffic_per_launch(name),
+            "algorithmic_flops_per_launch": v["flops"] / launches,
+            "algorithmic_bytes_per_launch": v["bytes"] / launches,
+            "avg_launch_us": 1e6 * sec, "launches_per_step": v["launches"] / steps,
+            "share_of_step": v["ms"] / steps}
+
+
+def cpu_baseline(g, perm, nsample):
+    """The CPU oracle (supernodal Cholesky, OpenMP) timed on the first
+    `nsample` systems of the same sequence with the same ordering; only its
+    numeric phase counts, as the GPU path's analysis is also done once."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from hykkt_amd.generator import ds_spread_schedule
+
+    oc = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
+    t_num, worst = 0.0, 0.0
+    for t in range(nsample):
+        sy = g.system(t, ds_range=ds_spread_schedule(t, 20))
+        r = O.execute(oc(sy.H), sy.Ds.v, oc(sy.Jc), oc(sy.Jd), sy.rx, sy.rs, sy.ry, sy.ryd,
+                      g.gamma, perm=perm, chol_kind=1)
+        t_num += r["stats"]["t_numeric"]
+        worst = max(worst, r["stats"]["resid4"])
+        print(f"bench: cpu oracle solve {t}: {r['stats']['t_numeric']:.2f} s "
+              f"resid4={r['stats']['resid4']:.2e}", file=sys.stderr, flush=True)
+    return {"value": nsample / t_num, "unit": "KKT solves/s", "cores": O.threads(),
+            "kind": "port",
+            "sample": f"{nsample} solves (steps 0..{nsample - 1}) of the same 20-step sequence, "
+                      "oracle/hykkt_oracle.c execute() with its supernodal left-looking "
+                      "Cholesky, same ordering, numeric phase only",
+            "max_resid4": worst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -36,6 +105,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="grid10k_seq20")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=3,
+                    help="solves of the sequence the CPU oracle is timed on")
+    ap.add_argument("--kernel-timing", type=int, default=1)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run the plain copy kernel over this many MiB (3 times), the "
+                         "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the KKT solve of one "
@@ -62,6 +137,17 @@ def main():
                Dsci=i32(np.arange(g.d)))
     H.device_synchronize()
 
+    if args.copy_calibration_mb > 0:
+        cnt = args.copy_calibration_mb * (1 << 20) // 8
+        src, dst = H.DeviceArray(np.ones(cnt)), H.DeviceArray(np.zeros(cnt))
+        for _ in range(3):
+            H.device_copy(dst, src)
+        H.device_synchronize()
+        print(f"bench: copy calibration k_copy {cnt} doubles = {16 * cnt} bytes moved per launch",
+              file=sys.stderr, flush=True)
+        src.free()
+        dst.free()
+
     s = H.HykktSolver(g.gamma)
     s.options(inputs_on_device=1)
 
@@ -86,7 +172,9 @@ def main():
               f"cg={st['t_cg_ms']:.1f}ms sym={st['t_symbolic_ms']:.0f}ms", file=sys.stderr,
               flush=True)
 
-    lat, worst, iters, tf, tcg = [], 0.0, [], [], []
+    lat, worst, iters, tf, tcg, nref, first = [], 0.0, [], [], [], [], 0.0
+    if args.kernel_timing:
+        s.profile(True)   # HIP events around every factor / solve launch
     H.device_synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -99,11 +187,15 @@ def main():
         iters.append(st["cg_iters"])
         tf.append(st["t_factor_ms"])
         tcg.append(st["t_cg_ms"])
+        nref.append(st["refine_steps"])
+        first = max(first, st["resid4_first"])
         if rc != 0 or not st["resid4"] <= 1e-8:
             print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
                   flush=True)
     H.device_synchronize()
     total = time.perf_counter() - t_start
+    kt = s.kernel_times() if args.kernel_timing else {}
+    s.profile(False)
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": metric_name(),
@@ -128,7 +220,19 @@ def main():
         "cg_iters_mean": float(np.mean(iters)),
         "factor_ms_mean": float(np.mean(tf)),
         "cg_ms_mean": float(np.mean(tcg)),
+        "refine_steps_mean": float(np.mean(nref)),
+        "max_resid4_before_refinement": first,
+        "roofline": roofline(kt, args.steps),
+        "kernels": {k: {"launches_per_step": v["launches"] / args.steps,
+                        "ms_per_step": v["ms"] / args.steps,
+                        "avg_us": 1e3 * v["ms"] / max(v["launches"], 1),
+                        "algorithmic_flops_per_launch": v["flops"] / max(v["launches"], 1),
+                        "algorithmic_bytes_per_launch": v["bytes"] / max(v["launches"], 1)}
+                    for k, v in kt.items()},
+        "cpu_baseline": None,
     }
+    if args.cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, s.permutation(), args.cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -41,7 +41,7 @@ class Options(C.Structure):
     _fields_ = [("ruiz_its", C.c_int), ("norm_tol", C.c_double), ("pivot_tol", C.c_double),
                 ("cg_tol", C.c_double), ("cg_itmax", C.c_int), ("delta", C.c_double),
                 ("inputs_on_device", C.c_int), ("cg_check_every", C.c_int),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -51,10 +51,17 @@ class Stats(C.Structure):
                 ("nnz_hgam", C.c_int), ("nnz_l", C.c_longlong), ("n_supernodes", C.c_int),
                 ("n_levels", C.c_int), ("max_front", C.c_int), ("factor_flops", C.c_double),
                 ("t_symbolic_ms", C.c_double), ("t_solve_ms", C.c_double),
-                ("t_factor_ms", C.c_double), ("t_cg_ms", C.c_double), ("n_solves", C.c_int)]
+                ("t_factor_ms", C.c_double), ("t_cg_ms", C.c_double), ("n_solves", C.c_int),
+                ("resid4_first", C.c_double), ("refine_steps", C.c_int),
+                ("t_refine_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_longlong), ("ms", C.c_double),
+                ("flops", C.c_double), ("bytes", C.c_double)]
 
 
 _vp = C.c_void_p
@@ -95,6 +102,11 @@ _SIGS = {
     "hykkt_memcpy_htod": ([_vp, _vp, C.c_ulonglong], C.c_int),
     "hykkt_memcpy_dtoh": ([_vp, _vp, C.c_ulonglong], C.c_int),
     "hykkt_device_synchronize": ([], C.c_int),
+    "hykkt_device_copy_f64": ([_vp, _vp, C.c_longlong], C.c_int),
+    "hykkt_mm_read_csr": ([C.c_char_p, C.c_int, C.c_int, _ip, _ip, _ip, _vp, _vp, _vp], C.c_int),
+    "hykkt_mm_read_rhs": ([C.c_char_p, _ip, _vp], C.c_int),
+    "hykkt_profile_enable": ([_vp, C.c_int], C.c_int),
+    "hykkt_profile_get": ([_vp, C.POINTER(KernelTime), C.c_int, _ip], C.c_int),
     "hykkt_ruiz_scale": ([C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                           _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int),
     "hykkt_perm_reverse": ([C.c_int, _vp, _vp], C.c_int),
@@ -260,6 +272,19 @@ class HykktSolver:
         _check(lib().hykkt_get_stats(self._h, C.byref(st)))
         return st.as_dict()
 
+    def profile(self, on=True):
+        """Start (and clear) or stop per-kernel timing with HIP events."""
+        _check(lib().hykkt_profile_enable(self._h, int(bool(on))))
+
+    def kernel_times(self) -> dict:
+        """{kernel: {launches, ms, flops, bytes}} summed since profile(True)."""
+        buf = (KernelTime * 16)()
+        cnt = C.c_int()
+        _check(lib().hykkt_profile_get(self._h, buf, 16, C.byref(cnt)))
+        return {buf[k].name.decode(): {"launches": buf[k].launches, "ms": buf[k].ms,
+                                       "flops": buf[k].flops, "bytes": buf[k].bytes}
+                for k in range(min(cnt.value, 16))}
+
     def solution(self):
         st = self.stats()
         n, m, d = st["n"], st["m"], st["d"]
@@ -321,6 +346,38 @@ class DeviceArray:
 
 def device_synchronize():
     _check(lib().hykkt_device_synchronize())
+
+
+def device_copy(dst: "DeviceArray", src: "DeviceArray"):
+    """dst <- src (float64) with the library's plain streaming copy kernel."""
+    n = src.nbytes // 8
+    assert dst.nbytes >= src.nbytes and src.dtype == np.float64
+    _check(lib().hykkt_device_copy_f64(dst.ptr, src.ptr, n))
+
+
+def mm_read_csr(path, skip_lines, symmetric):
+    """input_functions.cpp ingest: (n, m, rowptr, colidx, vals) of a
+    MatrixMarket file (symmetric: lower triangle expanded to both)."""
+    n, m, nnz = C.c_int(), C.c_int(), C.c_int()
+    p = str(path).encode()
+    _check(lib().hykkt_mm_read_csr(p, int(skip_lines), int(bool(symmetric)), C.byref(n),
+                                   C.byref(m), C.byref(nnz), None, None, None))
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(max(nnz.value, 1), np.int32)
+    v = np.zeros(max(nnz.value, 1))
+    _check(lib().hykkt_mm_read_csr(p, int(skip_lines), int(bool(symmetric)), C.byref(n),
+                                   C.byref(m), C.byref(nnz), _p(rp), _p(ci), _p(v)))
+    return n.value, m.value, rp, ci[:nnz.value], v[:nnz.value]
+
+
+def mm_read_rhs(path):
+    """input_functions.cpp read_rhs."""
+    n = C.c_int()
+    p = str(path).encode()
+    _check(lib().hykkt_mm_read_rhs(p, C.byref(n), None))
+    v = np.zeros(max(n.value, 1))
+    _check(lib().hykkt_mm_read_rhs(p, C.byref(n), _p(v)))
+    return v[:n.value]
 
 
 # ---------------------------------------------------------------- components

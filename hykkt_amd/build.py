@@ -28,6 +28,7 @@ HIP_SOURCES = {
     "kernels_trsv.hip": [],
 }
 CXX_SOURCES = ["symbolic.cpp", "amd.cpp", "cholplan.cpp", "mmio.cpp", "chol_engine.cpp",
+               "profiler.cpp",
                "cg.cpp", "solver.cpp", "capi.cpp"]
 
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -9,6 +9,7 @@
 
 #include "../../include/hykkt.h"
 #include "device.hpp"
+#include "mmio.hpp"
 #include "solver.hpp"
 
 struct hykkt_solver_s {
@@ -73,6 +74,8 @@ int hykkt_default_options(hykkt_options* o)
     o->inputs_on_device = 0;
     o->cg_check_every = 1;
     o->verbose = 0;
+    o->refine_max = 3;
+    o->refine_tol = 1e-9;
     return HYKKT_OK;
   });
 }
@@ -157,13 +160,43 @@ int hykkt_execute(hykkt_solver s)
   });
 }
 
+int hykkt_profile_enable(hykkt_solver s, int on)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.prof.enable(on != 0);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_profile_get(hykkt_solver s, hykkt_kernel_time* out, int capacity, int* count)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(count);
+    HYKKT_REQUIRE(capacity >= 0 && (out || capacity == 0), HYKKT_E_ARG, "bad output buffer");
+    *count = hykkt::PK_COUNT;
+    for (int k = 0; k < hykkt::PK_COUNT && k < capacity; k++) {
+      const auto& t = s->impl.prof.total(k);
+      std::memset(out[k].name, 0, sizeof(out[k].name));
+      std::strncpy(out[k].name, hykkt::Profiler::name(k), sizeof(out[k].name) - 1);
+      out[k].launches = t.launches;
+      out[k].ms = t.ms;
+      out[k].flops = t.flops;
+      out[k].bytes = t.bytes;
+    }
+    return HYKKT_OK;
+  });
+}
+
 int hykkt_set_options(hykkt_solver s, const hykkt_options* o)
 {
   return guard([&] {
     NEED(s);
     NEED(o);
-    HYKKT_REQUIRE(o->ruiz_its >= 0 && o->cg_itmax >= 0 && o->cg_check_every >= 0, HYKKT_E_ARG,
-                  "negative iteration count");
+    HYKKT_REQUIRE(o->ruiz_its >= 0 && o->cg_itmax >= 0 && o->cg_check_every >= 0 &&
+                      o->refine_max >= 0,
+                  HYKKT_E_ARG, "negative iteration count");
     bool dev_changed = o->inputs_on_device != s->impl.opts.inputs_on_device;
     s->impl.opts = *o;
     (void)dev_changed;
@@ -264,6 +297,50 @@ int hykkt_memcpy_htod(void* dst, const void* src, unsigned long long bytes)
 {
   return guard([&] {
     if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_mm_read_csr(const char* path, int skip_lines, int symmetric, int* n, int* m, int* nnz,
+                      int* rowptr, int* colidx, double* vals)
+{
+  return guard([&] {
+    NEED(path);
+    NEED(n);
+    NEED(m);
+    NEED(nnz);
+    hykkt::CsrHost c = hykkt::read_mm_csr(path, skip_lines, symmetric != 0);
+    *n = c.n;
+    *m = c.m;
+    *nnz = c.nnz();
+    if (rowptr) std::copy(c.rp.begin(), c.rp.end(), rowptr);
+    if (colidx) std::copy(c.ci.begin(), c.ci.end(), colidx);
+    if (vals) std::copy(c.v.begin(), c.v.end(), vals);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_mm_read_rhs(const char* path, int* n, double* vals)
+{
+  return guard([&] {
+    NEED(path);
+    NEED(n);
+    std::vector<double> v = hykkt::read_rhs(path);
+    *n = static_cast<int>(v.size());
+    if (vals) std::copy(v.begin(), v.end(), vals);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_device_copy_f64(double* dst, const double* src, long long count)
+{
+  return guard([&] {
+    HYKKT_REQUIRE(count >= 0, HYKKT_E_ARG, "negative count");
+    if (count) {
+      NEED(dst);
+      NEED(src);
+      hykkt::dev::copy(dst, src, count, nullptr);
+    }
     return HYKKT_OK;
   });
 }

@@ -113,6 +113,9 @@ void CholEngine::build_tasks()
           while (j < con.size() && con[j].t == con[i].t) {
             eacc.push_back(con[j].c);
             eaccol.push_back(con[j].r2);
+            const double e = pl_.nr(con[j].c) - con[j].r2;   // entries of that U column
+            la.flops += e;
+            la.bytes += 24.0 * e;   // read the child entry, read + write the parent's
             j++;
           }
           eap.push_back(p);
@@ -141,6 +144,18 @@ void CholEngine::build_tasks()
         potrf.push_back(s);
         L.potrf[j].maxb = std::max(L.potrf[j].maxb, b);
         const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
+        // Algorithmic (LAPACK) counts: POTRF b^3/3, TRSM r*b^2 for the r rows
+        // below the block, and the lower-trapezoid rank-b update of the c
+        // trailing panel columns; bytes read each operand and write each
+        // result once.
+        const double db = b, r = f - (k0 + b), cc = nc - (k0 + b);
+        const double tri = db * (db + 1) / 2, pairs = cc * r - cc * (cc - 1) / 2;
+        L.potrf[j].flops += db * db * db / 3 + db * db / 2 + db / 6;
+        L.potrf[j].bytes += 16.0 * tri;
+        L.trsm[j].flops += r * db * db;
+        L.trsm[j].bytes += 8.0 * (2 * r * db + tri);
+        L.upd[j].flops += 2.0 * db * pairs;
+        L.upd[j].bytes += 8.0 * (r * db + 2 * pairs);
         for (int t = 0; t < rtiles; t++) {
           trsm_sn.push_back(s);
           trsm_tr.push_back(t);
@@ -160,6 +175,9 @@ void CholEngine::build_tasks()
     L.syrk.off = static_cast<int>(syrk_sn.size());
     for (int q = b0; q < b1; q++) {
       const int s = pl_.level_sn[q], nt = cdiv(pl_.nr(s), NB);
+      const double nr = pl_.nr(s), nc = pl_.nc(s), pairs = nr * (nr + 1) / 2;
+      L.syrk.flops += 2.0 * nc * pairs;
+      L.syrk.bytes += 8.0 * (nr * nc + pairs);
       for (int tc = 0; tc < nt; tc++)
         for (int tr = tc; tr < nt; tr++) {
           syrk_sn.push_back(s);
@@ -268,21 +286,43 @@ void CholEngine::factor(double pivot_tol)
 {
   cd_.pivot_tol = pivot_tol;
   dev::fill_i(pivot_.p, INT_MAX, 1, st_);
+  Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
+  // Bracket one launch (skipped when it has no tasks, like the launchers).
+  auto timed = [&](int kind, const Launch& la, auto&& launch) {
+    if (la.cnt <= 0) return;
+    if (pf) pf->begin(st_);
+    launch();
+    if (pf) pf->end(kind, st_, la.flops, la.bytes);
+  };
   for (const Level& L : lv_) {
-    dev::extend_add(cd_, L.ea_panel.cnt, t_ea_p_.p + L.ea_panel.off, t_ea_col_.p + L.ea_panel.off,
-                    t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
+    timed(PK_EXTEND_ADD, L.ea_panel, [&] {
+      dev::extend_add(cd_, L.ea_panel.cnt, t_ea_p_.p + L.ea_panel.off,
+                      t_ea_col_.p + L.ea_panel.off, t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p,
+                      t_ea_ccol_.p, st_);
+    });
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
-      dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb, st_);
-      dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off, t_trsm_tr_.p + L.trsm[j].off,
-                    t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
-      dev::dense_nt(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,
-                    t_upd_tc_.p + L.upd[j].off, 0, step, st_);
+      timed(PK_POTRF, L.potrf[j], [&] {
+        dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb,
+                       st_);
+      });
+      timed(PK_DENSE_NT, L.trsm[j], [&] {
+        dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
+                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
+      });
+      timed(PK_DENSE_NT, L.upd[j], [&] {
+        dev::dense_nt(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,
+                      t_upd_tc_.p + L.upd[j].off, 0, step, st_);
+      });
     }
-    dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
-                  t_syrk_tc_.p + L.syrk.off, 1, 0, st_);
-    dev::extend_add(cd_, L.ea_u.cnt, t_ea_p_.p + L.ea_u.off, t_ea_col_.p + L.ea_u.off,
-                    t_ea_cptr_.p + L.ea_u.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
+    timed(PK_DENSE_NT, L.syrk, [&] {
+      dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
+                    t_syrk_tc_.p + L.syrk.off, 1, 0, st_);
+    });
+    timed(PK_EXTEND_ADD, L.ea_u, [&] {
+      dev::extend_add(cd_, L.ea_u.cnt, t_ea_p_.p + L.ea_u.off, t_ea_col_.p + L.ea_u.off,
+                      t_ea_cptr_.p + L.ea_u.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
+    });
   }
 }
 
@@ -306,8 +346,19 @@ int CholEngine::solve_error()
 void CholEngine::solve(double* x)
 {
   dev::solve_reset(sd_, pl_.ns, st_);
+  Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
+  // Algorithmic: each sweep reads the true factor once, 2 flops per entry,
+  // and reads + writes the vector.
+  const double fl = 2.0 * static_cast<double>(pl_.nnz_l_true);
+  const double by = 8.0 * (static_cast<double>(pl_.nnz_l_true) + 2.0 * pl_.n);
+  if (pf) pf->begin(st_);
   dev::solve_forward(cd_, sd_, x, st_);
+  if (pf) {
+    pf->end(PK_SOLVE_FWD, st_, fl, by);
+    pf->begin(st_);
+  }
   dev::solve_backward(cd_, sd_, x, st_);
+  if (pf) pf->end(PK_SOLVE_BWD, st_, fl, by);
 }
 
 }  // namespace hykkt

@@ -4,10 +4,12 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "profiler.hpp"
 #include "symbolic.hpp"
 
 namespace hykkt {
@@ -20,6 +22,12 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
+  // Exchange the buffers (no copies, no allocation).
+  void swap(DevBuf& o) noexcept
+  {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
   void alloc(size_t cnt)
   {
     release();
@@ -76,10 +84,13 @@ class CholEngine {
   const int64_t* a_dst() const { return a_dst_.p; }
   int n_assembled() const { return static_cast<int>(pl_.a_src.size()); }
   const int* post_dev() const { return post_.p; }
+  // Time every factor/solve launch with `p` (nullptr: no timing).
+  void set_profiler(Profiler* p) { prof_ = p; }
 
  private:
   struct Launch {
     int off = 0, cnt = 0, maxb = 0;
+    double flops = 0.0, bytes = 0.0;   // algorithmic, for the profiler
   };
   struct Level {
     Launch ea_panel, ea_u, syrk;
@@ -90,6 +101,7 @@ class CholEngine {
 
   CholPlan pl_;
   hipStream_t st_;
+  Profiler* prof_ = nullptr;
   dev::CholDev cd_{};
   dev::SolveDev sd_{};
   DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_, sn_parent_;

@@ -24,7 +24,8 @@ void gather_scale(double* dst, const double* src, const int* map, const double* 
 void vec_scale(double* y, const double* s, int n, hipStream_t st);
 
 // ---------------------------------------------------------------- sparse
-// jd_vs = Ds-row-scaled Jd; ryd_s = ryd*ds + rs  (row_scale :490-509, add_vecs :310-317)
+// jd_vs = Ds-row-scaled Jd (skipped if jd_vs is null); ryd_s = ryd*ds + rs
+// (row_scale :490-509, add_vecs :310-317)
 void row_scale(int d, const int* jd_rp, const double* jd_v, const double* ds,
                const double* ryd, const double* rs, double* jd_vs, double* ryd_s, hipStream_t st);
 // y_out[i] = alpha*t + beta*y_in[i] (beta == 0: alpha*t), t = sum fma(v[map?map[q]:q], x[ci[q]])
@@ -78,12 +79,18 @@ void resid_rows_n(int n, const int* h_rp, const int* h_ci, const double* h_v,
                   const int* jct_rp, const int* jct_ci, const int* jct_map, const double* jc_v,
                   const int* jdt_rp, const int* jdt_ci, const int* jdt_map, const double* jd_v,
                   const double* rx, const double* dx, const double* dyc, const double* dyd,
-                  double* sq, hipStream_t st);
+                  double* sq, double* e, hipStream_t st);
+//   m rows: r3 = ry - Jc dx
 void resid_rows_m(int m, const int* jc_rp, const int* jc_ci, const double* jc_v, const double* ry,
-                  const double* dx, double* sq, hipStream_t st);
+                  const double* dx, double* sq, double* e, hipStream_t st);
+//   d rows: r2 = rs - (Ds ds - dyd), r4 = ryd - (Jd dx - ds)
+// e, e2, e4 (may be null): the signed residual rows, for iterative refinement.
 void resid_rows_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, int has_jd,
                   const double* ds, const double* rs, const double* ryd, const double* dx,
-                  const double* dsv, const double* dyd, double* sq2, double* sq4, hipStream_t st);
+                  const double* dsv, const double* dyd, double* sq2, double* sq4, double* e2,
+                  double* e4, hipStream_t st);
+// out = a + b
+void add(double* out, const double* a, const double* b, int64_t n, hipStream_t st);
 // ds = Jd dx - ryd (or -ryd); dyd = ds*Ds - rs
 void recover_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, int has_jd,
                const double* dx, const double* ryd, const double* ds_diag, const double* rs,

@@ -86,7 +86,8 @@ __global__ void k_row_scale(int d, const int* __restrict__ rp, const double* __r
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d) return;
   double di = ds[i];
-  for (int p = rp[i]; p < rp[i + 1]; p++) vs[p] = v[p] * di;
+  if (vs)
+    for (int p = rp[i]; p < rp[i + 1]; p++) vs[p] = v[p] * di;
   double t = ryd[i] * di;
   ryd_s[i] = t + 1.0 * rs[i];
 }
@@ -286,7 +287,8 @@ __global__ void k_resid_n(int n, const int* __restrict__ h_rp, const int* __rest
                           const int* __restrict__ dt_ci, const int* __restrict__ dt_map,
                           const double* __restrict__ jd_v, const double* __restrict__ rx,
                           const double* __restrict__ dx, const double* __restrict__ dyc,
-                          const double* __restrict__ dyd, double* __restrict__ sq)
+                          const double* __restrict__ dyd, double* __restrict__ sq,
+                          double* __restrict__ e)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -296,17 +298,20 @@ __global__ void k_resid_n(int n, const int* __restrict__ h_rp, const int* __rest
   if (dt_rp)
     for (int q = dt_rp[i]; q < dt_rp[i + 1]; q++) t = fma(-jd_v[dt_map[q]], dyd[dt_ci[q]], t);
   sq[i] = t * t;
+  if (e) e[i] = t;
 }
 
 __global__ void k_resid_m(int m, const int* __restrict__ rp, const int* __restrict__ ci,
                           const double* __restrict__ v, const double* __restrict__ ry,
-                          const double* __restrict__ dx, double* __restrict__ sq)
+                          const double* __restrict__ dx, double* __restrict__ sq,
+                          double* __restrict__ e)
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= m) return;
   double t = ry[k];
   for (int p = rp[k]; p < rp[k + 1]; p++) t = fma(-v[p], dx[ci[p]], t);
   sq[k] = t * t;
+  if (e) e[k] = t;
 }
 
 __global__ void k_resid_d(int d, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -314,7 +319,8 @@ __global__ void k_resid_d(int d, const int* __restrict__ rp, const int* __restri
                           const double* __restrict__ ds, const double* __restrict__ rs,
                           const double* __restrict__ ryd, const double* __restrict__ dx,
                           const double* __restrict__ dsv, const double* __restrict__ dyd,
-                          double* __restrict__ sq2, double* __restrict__ sq4)
+                          double* __restrict__ sq2, double* __restrict__ sq4,
+                          double* __restrict__ e2, double* __restrict__ e4)
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d) return;
@@ -325,6 +331,16 @@ __global__ void k_resid_d(int d, const int* __restrict__ rp, const int* __restri
   double a4 = ryd[k] - (jx - dsv[k]);
   sq2[k] = a2 * a2;
   sq4[k] = a4 * a4;
+  if (e2) e2[k] = a2;
+  if (e4) e4[k] = a4;
+}
+
+__global__ void k_add(double* __restrict__ out, const double* __restrict__ a,
+                      const double* __restrict__ b, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] + b[i];
 }
 
 __global__ void k_recover_d(int d, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -505,29 +521,39 @@ void resid_rows_n(int n, const int* h_rp, const int* h_ci, const double* h_v, co
                   const int* ct_ci, const int* ct_map, const double* jc_v, const int* dt_rp,
                   const int* dt_ci, const int* dt_map, const double* jd_v, const double* rx,
                   const double* dx, const double* dyc, const double* dyd, double* sq,
-                  hipStream_t st)
+                  double* e, hipStream_t st)
 {
   if (n <= 0) return;
   k_resid_n<<<nblocks(n), TPB, 0, st>>>(n, h_rp, h_ci, h_v, ct_rp, ct_ci, ct_map, jc_v, dt_rp,
-                                        dt_ci, dt_map, jd_v, rx, dx, dyc, dyd, sq);
+                                        dt_ci, dt_map, jd_v, rx, dx, dyc, dyd, sq, e);
   HIPCHK(hipGetLastError());
 }
 
 void resid_rows_m(int m, const int* rp, const int* ci, const double* v, const double* ry,
-                  const double* dx, double* sq, hipStream_t st)
+                  const double* dx, double* sq, double* e, hipStream_t st)
 {
   if (m <= 0) return;
-  k_resid_m<<<nblocks(m), TPB, 0, st>>>(m, rp, ci, v, ry, dx, sq);
+  k_resid_m<<<nblocks(m), TPB, 0, st>>>(m, rp, ci, v, ry, dx, sq, e);
   HIPCHK(hipGetLastError());
 }
 
 void resid_rows_d(int d, const int* rp, const int* ci, const double* v, int has_jd,
                   const double* ds, const double* rs, const double* ryd, const double* dx,
-                  const double* dsv, const double* dyd, double* sq2, double* sq4, hipStream_t st)
+                  const double* dsv, const double* dyd, double* sq2, double* sq4, double* e2,
+                  double* e4, hipStream_t st)
 {
   if (d <= 0) return;
   k_resid_d<<<nblocks(d), TPB, 0, st>>>(d, rp, ci, v, has_jd, ds, rs, ryd, dx, dsv, dyd, sq2,
-                                        sq4);
+                                        sq4, e2, e4);
+  HIPCHK(hipGetLastError());
+}
+
+void add(double* out, const double* a, const double* b, int64_t n, hipStream_t st)
+{
+  if (n <= 0) return;
+  unsigned g = nblocks(n);
+  if (g > 65536) g = 65536;
+  k_add<<<g, TPB, 0, st>>>(out, a, b, n);
   HIPCHK(hipGetLastError());
 }
 

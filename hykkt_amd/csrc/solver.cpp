@@ -38,6 +38,9 @@ struct Solver::Sym {
   DevBuf<double> jch_v, jcth_v, bhat, rxp, tmp_n, t1;
   DevBuf<double> cx, cr, cw, cp, cs, schur, sc, part;
   DevBuf<double> dx, dsv, dyc, dyd, sq_n, sq_m, sq_d2, sq_d4, res;
+  // refinement: signed residual rows, scaled rhs, correction / candidate
+  DevBuf<double> e_x, e_s, e_y, e_d, e_rxt, e_ysc, e_rxp, e_schur, e_cx;
+  DevBuf<double> c_dx, c_dsv, c_dyc, c_dyd;
   // current value pointers (device)
   const double *hv = nullptr, *jcv = nullptr, *jdv = nullptr, *dsd = nullptr;
   const double *drx = nullptr, *drs = nullptr, *dry = nullptr, *dryd = nullptr;
@@ -233,6 +236,7 @@ void Solver::analyze()
   perm_map_r(S->jct.at, S->perm, S->jctp_pat, S->jctp_map);
   // Cholesky analysis of the lower triangle of P Hg P^T (CholeskyClass.cpp:31-58)
   S->chol = std::make_unique<CholEngine>(S->hgp_pat, CholOptions{}, st_);
+  S->chol->set_profiler(&prof);
   const CholPlan& pl = S->chol->plan();
   S->phat.resize(n);
   S->ipos.resize(n);
@@ -311,6 +315,9 @@ void Solver::analyze()
   A(S->sq_d2, d);
   A(S->sq_d4, d);
   A(S->res, 16);
+  for (DevBuf<double>* b : {&S->e_x, &S->e_rxt, &S->e_rxp, &S->c_dx}) A(*b, n);
+  for (DevBuf<double>* b : {&S->e_y, &S->e_ysc, &S->e_schur, &S->e_cx, &S->c_dyc}) A(*b, m);
+  for (DevBuf<double>* b : {&S->e_s, &S->e_d, &S->c_dsv, &S->c_dyd}) A(*b, d);
   if (!on_dev) {
     A(S->h_v, S->nnz_h);
     A(S->jc_v, S->nnz_jc);
@@ -445,20 +452,113 @@ int Solver::execute()
   ch.factor(opts.pivot_tol);
   HIPCHK(hipEventRecord(ev_[1], st_));
 
-  // permuted Jc and Jc^T values, rhs r^x in internal order
+  // permuted Jc and Jc^T values (apply_permutation, HykktSolver.cpp:386-395)
   dev::gather(S.jch_v.p, S.jcs.p, S.jch_map.p, S.nnz_jc, st_);
   dev::gather(S.jcth_v.p, S.jcs.p, S.jcth_map.p, S.nnz_jc, st_);
-  dev::gather(S.tmp_n.p, S.rx_til.p, S.phat_d.p, n, st_);
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, gam, S.ry_s.p, 1.0, S.tmp_n.p,
-            S.rxp.p, st_);
 
-  // ---- setup_conjugate_gradient (HykktSolver.cpp:814-843) ----
-  dev::copy(S.t1.p, S.rxp.p, n, st_);
+  // ---- first pass: the reference's solve (HykktSolver.cpp:404-457) ----
+  Pass first{S.rxp.p, S.schur.p, S.cx.p};
+  int iters = 0, cg_status = 0;
+  double fnorm = 0.0;
+  solve_phase(first, S.rx_til.p, S.ry_s.p, S.drs, S.dryd, S.y0.p, S.dx.p, S.dsv.p, S.dyc.p,
+              S.dyd.p, &iters, &fnorm, &cg_status, ev_[2]);
+
+  // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
+  double hr[8];
+  residual(S.dx.p, S.dsv.p, S.dyc.p, S.dyd.p, true, ev_[3], hr);
+  const int hpiv = ch.pivot();
+  HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
+                "triangular-solve sweep exceeded its wait bound (results invalid)");
+  const double nb = hr[4] + hr[5] + hr[6] + hr[7];
+  stats.resid2 = std::sqrt(hr[0] + hr[1]) / std::sqrt(nb);
+  stats.resid4 = std::sqrt(hr[0] + hr[1] + hr[2] + hr[3]) / std::sqrt(nb);
+  stats.resid4_first = stats.resid4;
+  stats.refine_steps = 0;
+
+  // ---- iterative refinement on the 4x4 block KKT system (an addition: the
+  // reference stops after one pass).  The signed residual rows go through the
+  // right-hand-side half of formation (r~x = ex + Jd^T (Ds ed + es), then the
+  // Ruiz scaling max_d) and the same solve phase with a zero CG start; a
+  // correction is kept only if it lowers resid4. ----
+  Pass corr{S.e_rxp.p, S.e_schur.p, S.e_cx.p};
+  for (int k = 0; hpiv < 0 && k < opts.refine_max && stats.resid4 > opts.refine_tol; k++) {
+    if (S.has_jd) {
+      dev::row_scale(d, S.jd_rp.p, S.jdv, S.dsd, S.e_d.p, S.e_s.p, nullptr, S.ryd_s.p, st_);
+      dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, S.jdv, 1.0, S.ryd_s.p, 1.0, S.e_x.p,
+                S.e_rxt.p, st_);
+    } else {
+      dev::copy(S.e_rxt.p, S.e_x.p, n, st_);
+    }
+    dev::vec_scale(S.e_rxt.p, S.max_d.p, n, st_);
+    dev::copy(S.e_ysc.p, S.e_y.p, m, st_);
+    dev::vec_scale(S.e_ysc.p, S.max_d.p + n, m, st_);
+    int it = 0, cgs = 0;
+    double fn = 0.0;
+    solve_phase(corr, S.e_rxt.p, S.e_ysc.p, S.e_s.p, S.e_d.p, nullptr, S.c_dx.p, S.c_dsv.p,
+                S.c_dyc.p, S.c_dyd.p, &it, &fn, &cgs, nullptr);
+    dev::add(S.c_dx.p, S.c_dx.p, S.dx.p, n, st_);
+    dev::add(S.c_dsv.p, S.c_dsv.p, S.dsv.p, d, st_);
+    dev::add(S.c_dyc.p, S.c_dyc.p, S.dyc.p, m, st_);
+    dev::add(S.c_dyd.p, S.c_dyd.p, S.dyd.p, d, st_);
+    double cr[8];
+    residual(S.c_dx.p, S.c_dsv.p, S.c_dyc.p, S.c_dyd.p, true, nullptr, cr);
+    HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
+                  "triangular-solve sweep exceeded its wait bound (results invalid)");
+    const double r4 = std::sqrt(cr[0] + cr[1] + cr[2] + cr[3]) / std::sqrt(nb);
+    if (!(r4 < stats.resid4)) break;   // keep the better solution; the rows in e_* are stale
+    S.dx.swap(S.c_dx);
+    S.dsv.swap(S.c_dsv);
+    S.dyc.swap(S.c_dyc);
+    S.dyd.swap(S.c_dyd);
+    stats.resid2 = std::sqrt(cr[0] + cr[1]) / std::sqrt(nb);
+    stats.resid4 = r4;
+    stats.refine_steps++;
+  }
+  HIPCHK(hipEventRecord(ev_[4], st_));
+  HIPCHK(hipEventSynchronize(ev_[4]));
+  prof.collect();
+
+  stats.cg_iters = iters;
+  stats.cg_status = cg_status;
+  stats.cg_final_norm = fnorm;
+  stats.pivot = hpiv;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[4]));
+  stats.t_solve_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+  stats.t_factor_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
+  stats.t_cg_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[3], ev_[4]));
+  stats.t_refine_ms = ms;
+  stats.n_solves++;
+  if (opts.verbose) {
+    std::printf("||Ax-b||/||b|| = %32.32g\n\n", stats.resid2);
+    std::printf(stats.resid2 < opts.norm_tol ? "Residual test passed\n" : "Residual test failed\n");
+  }
+  if (hpiv >= 0) return HYKKT_RESIDUAL_FAIL;
+  return (stats.resid2 < opts.norm_tol) ? HYKKT_OK : HYKKT_RESIDUAL_FAIL;
+}
+
+// setup_conjugate_gradient (HykktSolver.cpp:814-843), the Schur CG
+// (SchurComplementConjugateGradient.cpp:89-194) and recover_solution
+// (HykktSolver.cpp:410-457) for Ruiz-scaled right-hand sides rxt (n, natural
+// order) and ysc (m); rs, ryd are the unscaled d-rows used by the recovery.
+void Solver::solve_phase(const Pass& w, const double* rxt, const double* ysc, const double* rs,
+                         const double* ryd, const double* x0, double* dx, double* dsv,
+                         double* dyc, double* dyd, int* iters, double* fnorm, int* cg_status,
+                         hipEvent_t after_cg)
+{
+  Sym& S = *sym_;
+  const int n = S.n, m = S.m, d = S.d;
+  // r^x = r~x + gamma Jc^T ry in the factor's internal order
+  dev::gather(S.tmp_n.p, rxt, S.phat_d.p, n, st_);
+  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, gamma_, ysc, 1.0, S.tmp_n.p, w.rxp,
+            st_);
+  dev::copy(S.t1.p, w.rxp, n, st_);
   chol_solve(S.t1.p);
-  dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, S.jch_v.p, 1.0, S.t1.p, -1.0, S.ry_s.p,
-            S.schur.p, st_);
+  dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, S.jch_v.p, 1.0, S.t1.p, -1.0, ysc, w.schur, st_);
 
-  // ---- SchurComplementConjugateGradient::solve (…ConjugateGradient.cpp:89-194) ----
   SchurOps ops;
   ops.m = m;
   ops.n = n;
@@ -471,37 +571,46 @@ int Solver::execute()
   ops.chol = S.chol.get();
   ops.st = st_;
   ops.t1 = S.t1.p;
-  ops.x = S.cx.p;
+  ops.x = w.cx;
   ops.r = S.cr.p;
   ops.w = S.cw.p;
   ops.p = S.cp.p;
   ops.s = S.cs.p;
   ops.sc = S.sc.p;
   ops.part = S.part.p;
-  dev::copy(S.cx.p, S.y0.p, m, st_);
-  int iters = 0;
-  double fnorm = 0.0;
-  int cg_status = schur_cg_run(ops, S.schur.p, opts.cg_tol, opts.cg_itmax, opts.cg_check_every,
-                               &iters, &fnorm);
-  HIPCHK(hipEventRecord(ev_[2], st_));
+  if (x0) dev::copy(w.cx, x0, m, st_);
+  else dev::fill(w.cx, 0.0, m, st_);
+  *cg_status =
+      schur_cg_run(ops, w.schur, opts.cg_tol, opts.cg_itmax, opts.cg_check_every, iters, fnorm);
+  if (after_cg) HIPCHK(hipEventRecord(after_cg, st_));
 
-  // ---- recover_solution (HykktSolver.cpp:410-457) ----
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, -1.0, S.cx.p, 1.0, S.rxp.p, S.t1.p,
+  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, -1.0, w.cx, 1.0, w.rxp, S.t1.p,
             st_);
   chol_solve(S.t1.p);
-  dev::gather_scale(S.dx.p, S.t1.p, S.ipos_d.p, S.max_d.p, n, st_);
-  dev::copy(S.dyc.p, S.cx.p, m, st_);
-  dev::vec_scale(S.dyc.p, S.max_d.p + n, m, st_);
-  dev::recover_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, S.dx.p, S.dryd, S.dsd, S.drs,
-                 S.dsv.p, S.dyd.p, st_);
+  dev::gather_scale(dx, S.t1.p, S.ipos_d.p, S.max_d.p, n, st_);
+  dev::copy(dyc, w.cx, m, st_);
+  dev::vec_scale(dyc, S.max_d.p + n, m, st_);
+  dev::recover_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, dx, ryd, S.dsd, rs, dsv, dyd,
+                 st_);
+}
 
-  // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
+// KKT residual of (dx, dsv, dyc, dyd) against the bound right-hand sides:
+// out[0..3] = squared norms of rows 1, 3, 2, 4; out[4..7] = squared norms of
+// rx, rs, ry, ryd.  rows: also keep the signed rows in e_x, e_y, e_s, e_d.
+// Synchronizes the stream.
+void Solver::residual(const double* dx, const double* dsv, const double* dyc, const double* dyd,
+                      bool rows, hipEvent_t done, double out[8])
+{
+  Sym& S = *sym_;
+  const int n = S.n, m = S.m, d = S.d;
   dev::resid_rows_n(n, S.h_rp.p, S.h_ci.p, S.hv, S.jct_rp.p, S.jct_ci.p, S.jct_map.p, S.jcv,
-                    S.has_jd ? S.jdt_rp.p : nullptr, S.jdt_ci.p, S.jdt_map.p, S.jdv, S.drx,
-                    S.dx.p, S.dyc.p, S.dyd.p, S.sq_n.p, st_);
-  dev::resid_rows_m(m, S.jc_rp.p, S.jc_ci.p, S.jcv, S.dry, S.dx.p, S.sq_m.p, st_);
-  dev::resid_rows_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, S.dsd, S.drs, S.dryd,
-                    S.dx.p, S.dsv.p, S.dyd.p, S.sq_d2.p, S.sq_d4.p, st_);
+                    S.has_jd ? S.jdt_rp.p : nullptr, S.jdt_ci.p, S.jdt_map.p, S.jdv, S.drx, dx,
+                    dyc, dyd, S.sq_n.p, rows ? S.e_x.p : nullptr, st_);
+  dev::resid_rows_m(m, S.jc_rp.p, S.jc_ci.p, S.jcv, S.dry, dx, S.sq_m.p,
+                    rows ? S.e_y.p : nullptr, st_);
+  dev::resid_rows_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, S.dsd, S.drs, S.dryd, dx,
+                    dsv, dyd, S.sq_d2.p, S.sq_d4.p, rows ? S.e_s.p : nullptr,
+                    rows ? S.e_d.p : nullptr, st_);
   double* r = S.res.p;
   dev::dot(n, S.sq_n.p, nullptr, S.part.p, r + 0, st_);
   dev::dot(m, S.sq_m.p, nullptr, S.part.p, r + 1, st_);
@@ -511,35 +620,9 @@ int Solver::execute()
   dev::dot(d, S.drs, S.drs, S.part.p, r + 5, st_);
   dev::dot(m, S.dry, S.dry, S.part.p, r + 6, st_);
   dev::dot(d, S.dryd, S.dryd, S.part.p, r + 7, st_);
-  HIPCHK(hipEventRecord(ev_[3], st_));
-  double hr[8];
-  int hpiv = INT_MAX;
-  HIPCHK(hipMemcpyAsync(hr, r, 8 * sizeof(double), hipMemcpyDeviceToHost, st_));
+  if (done) HIPCHK(hipEventRecord(done, st_));
+  HIPCHK(hipMemcpyAsync(out, r, 8 * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  hpiv = ch.pivot();
-  HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
-                "triangular-solve sweep exceeded its wait bound (results invalid)");
-  double nb = hr[4] + hr[5] + hr[6] + hr[7];
-  stats.resid2 = std::sqrt(hr[0] + hr[1]) / std::sqrt(nb);
-  stats.resid4 = std::sqrt(hr[0] + hr[1] + hr[2] + hr[3]) / std::sqrt(nb);
-  stats.cg_iters = iters;
-  stats.cg_status = cg_status;
-  stats.cg_final_norm = fnorm;
-  stats.pivot = hpiv;
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[3]));
-  stats.t_solve_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-  stats.t_factor_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
-  stats.t_cg_ms = ms;
-  stats.n_solves++;
-  if (opts.verbose) {
-    std::printf("||Ax-b||/||b|| = %32.32g\n\n", stats.resid2);
-    std::printf(stats.resid2 < opts.norm_tol ? "Residual test passed\n" : "Residual test failed\n");
-  }
-  if (hpiv >= 0) return HYKKT_RESIDUAL_FAIL;
-  return (stats.resid2 < opts.norm_tol) ? HYKKT_OK : HYKKT_RESIDUAL_FAIL;
 }
 
 void Solver::get_solution(double* dx, double* ds, double* dyc, double* dyd)

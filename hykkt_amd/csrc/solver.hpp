@@ -42,6 +42,7 @@ class Solver {
 
   hykkt_options opts;
   hykkt_stats stats{};
+  Profiler prof;   // per-kernel timing of the factor and solve launches
 
  private:
   struct Sym;   // once-per-pattern state
@@ -49,12 +50,21 @@ class Solver {
   bool pattern_matches() const;
   void upload_values();
   void chol_solve(double* x);
+  // Work buffers of one solve pass (r^x permuted, Schur rhs, CG iterate).
+  struct Pass {
+    double *rxp, *schur, *cx;
+  };
+  void solve_phase(const Pass& w, const double* rxt, const double* ysc, const double* rs,
+                   const double* ryd, const double* x0, double* dx, double* dsv, double* dyc,
+                   double* dyd, int* iters, double* fnorm, int* cg_status, hipEvent_t after_cg);
+  void residual(const double* dx, const double* dsv, const double* dyc, const double* dyd,
+                bool rows, hipEvent_t done, double out[8]);
   const double* val(int which) const;
   const double* vec(int which) const;
 
   double gamma_;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev_[4] = {};
+  hipEvent_t ev_[5] = {};
   BlockIn blk_[4];
   const double* vecs_[8] = {};
   int vec_n_[8] = {};

@@ -110,6 +110,10 @@ typedef struct {
   int    inputs_on_device;  /* 1: pointers given to the setters are device pointers */
   int    cg_check_every;    /* CG convergence read back every k iterations (default 1) */
   int    verbose;           /* print the reference's progress lines (default 0) */
+  int    refine_max;        /* iterative-refinement passes on the 4x4 KKT system after the
+                               reference's single pass (default 3; 0 = the reference) */
+  double refine_tol;        /* refine while resid4 > refine_tol (default 1e-9); a pass is
+                               kept only if it lowers resid4 */
 } hykkt_options;
 
 typedef struct {
@@ -129,6 +133,9 @@ typedef struct {
   double t_factor_ms;       /* numeric factorization part of t_solve_ms */
   double t_cg_ms;           /* Schur CG part of t_solve_ms */
   int    n_solves;
+  double resid4_first;      /* resid4 after the first pass, before refinement */
+  int    refine_steps;      /* refinement passes kept in the last execute */
+  double t_refine_ms;       /* refinement part of t_solve_ms */
 } hykkt_stats;
 
 int hykkt_default_options(hykkt_options* o);
@@ -165,6 +172,17 @@ int hykkt_set_permutation(hykkt_solver s, const int* perm, int n);
  * Returns the number of elements copied or a negative error. */
 long long hykkt_debug_copy(hykkt_solver s, int what, void* host_out, long long capacity);
 
+/* MatrixMarket ingest with the reference's semantics: read_mm_file_into_coo
+ * followed by sym_coo_to_csr (symmetric = 1: a lower-triangle file expanded
+ * to both triangles) or coo_to_csr (input_functions.cpp:16-200); the last of
+ * the skip_lines skipped lines is the "rows cols nnz" line.  Call once with
+ * NULL arrays to learn n, m and nnz, then with arrays of n+1, nnz, nnz. */
+int hykkt_mm_read_csr(const char* path, int skip_lines, int symmetric, int* n, int* m, int* nnz,
+                      int* rowptr, int* colidx, double* vals);
+/* read_rhs (input_functions.cpp:202-217): two header lines, then one value
+ * per line.  NULL vals: only *n is set. */
+int hykkt_mm_read_rhs(const char* path, int* n, double* vals);
+
 /* Device memory helpers for callers without a HIP runtime of their own (for
  * example a ctypes binding), so that inputs can be made resident in HBM. */
 int hykkt_device_malloc(void** ptr, unsigned long long bytes);
@@ -172,6 +190,31 @@ int hykkt_device_free(void* ptr);
 int hykkt_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
 int hykkt_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
 int hykkt_device_synchronize(void);
+/* dst[i] = src[i], i < count, by a plain streaming copy kernel (null stream):
+ * the known-size reference against which HBM byte counters are scaled. */
+int hykkt_device_copy_f64(double* dst, const double* src, long long count);
+
+/* Per-kernel timing of the factorization and triangular-solve launches
+ * (no counterpart in the reference, which times whole phases with
+ * std::chrono in its examples).  While enabled, every launch of the kernels
+ * below is bracketed by HIP events on the solver's stream and its duration
+ * is summed with the algorithmic flops and HBM bytes it performs:
+ *   k_dense_nt    MFMA tiles: TRSM (r*b^2), panel update and U = -L21 L21^T
+ *                 (2*K per lower-triangle entry)
+ *   k_potrf_inv   128-column diagonal blocks (b^3/3)
+ *   k_extend_add  update-matrix assembly (1 flop, 24 bytes per entry)
+ *   k_solve_fwd / k_solve_bwd  one sweep each (2*nnz(L) flops, 8*nnz(L) bytes)
+ * hykkt_profile_enable clears the totals; hykkt_profile_get writes up to
+ * `capacity` entries and sets *count to the number of kernels. */
+typedef struct {
+  char name[32];
+  long long launches;
+  double ms;      /* sum of launch durations */
+  double flops;   /* algorithmic flops of those launches */
+  double bytes;   /* algorithmic HBM bytes of those launches */
+} hykkt_kernel_time;
+int hykkt_profile_enable(hykkt_solver s, int on);
+int hykkt_profile_get(hykkt_solver s, hykkt_kernel_time* out, int capacity, int* count);
 
 /* ---------------------------------------------------------------------------
  * Component entry points mirroring the reference's step classes, used by

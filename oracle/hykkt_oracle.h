@@ -14,8 +14,10 @@
  *   orc_map_rc/_c/_r, orc_perm_rev                      pinned: perm_driver.cpp KAT + _ref.
  *   orc_transpose, orc_spgemm_*                         pinned: dense brute force (tests).
  *   orc_ruiz                                           pinned: Ruiz_driver.cpp KAT (6 values).
- *   orc_chol_*                                         pinned: cuSolver_driver_cholesky.cpp KAT
+ *   orc_chol_* (kind 0, up-looking)                    pinned: cuSolver_driver_cholesky.cpp KAT
  *        + dense numpy Cholesky / residual at small n.
+ *   orc_chol_* (kind 1, supernodal)                    pinned: same KAT + agreement with
+ *        kind 0 (solutions, nnz(L), flop count) on sparse SPD systems up to n = 4,000.
  *   orc_schur_cg                                       pinned: cuSolver_driver_schur_cg.cpp KAT
  *        + dense direct Schur solve.
  *   orc_execute                                        pinned: dense LU of the full 4x4 block
@@ -27,6 +29,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+int  orc_threads(void);   /* OpenMP threads used by the parallel loops */
 
 /* ---------------- MatrixMarket ingest (src/input_functions.cpp) ---------------- */
 int  orc_mm_header(const char* path, int skip, int* n, int* m, int* nnz);
@@ -69,6 +73,9 @@ void orc_map_r(int n, const int* rows, const int* cols, const int* perm, int* pr
 /* ---------------- sparse Cholesky of the lower triangle ---------------- */
 typedef struct orc_chol orc_chol;
 orc_chol* orc_chol_create(int n, const int* rp, const int* ci);
+/* kind 0: up-looking (the form above); kind 1: supernodal left-looking on
+ * OpenMP threads -- same factor to rounding, for the large systems. */
+orc_chol* orc_chol_create_kind(int n, const int* rp, const int* ci, int kind);
 int       orc_chol_factor(orc_chol* c, const double* vals, double tol);
 void      orc_chol_solve(const orc_chol* c, const double* b, double* x);
 long long orc_chol_nnz(const orc_chol* c);
@@ -90,6 +97,9 @@ typedef struct {
   int    cg_itmax;
   double pivot_tol;
   double norm_tol;
+  int    chol_kind;   /* orc_chol_create_kind() kind */
+  int    refine_max;  /* iterative-refinement passes at most (0: none, as the reference) */
+  double refine_tol;  /* refine while the 4-row residual is above this */
 } orc_opts;
 
 typedef struct {
@@ -101,6 +111,8 @@ typedef struct {
   int    nnz_htil;
   int    nnz_hgam;
   long long nnz_l;
+  double resid4_first; /* 4-row residual before refinement */
+  int    refine_steps;  /* refinement passes kept */
   double t_symbolic;  /* seconds */
   double t_numeric;   /* seconds: everything a repeated solve does */
 } orc_stats;

@@ -48,6 +48,8 @@ def lib():
         L.orc_map_r.argtypes = [C.c_int, _i, _i, _i, _i, _i, _i]
         L.orc_chol_create.argtypes = [C.c_int, _i, _i]
         L.orc_chol_create.restype = vp
+        L.orc_chol_create_kind.argtypes = [C.c_int, _i, _i, C.c_int]
+        L.orc_chol_create_kind.restype = vp
         L.orc_chol_factor.argtypes = [vp, _d, C.c_double]
         L.orc_chol_solve.argtypes = [vp, _d, _d]
         L.orc_chol_nnz.argtypes = [vp]
@@ -80,14 +82,15 @@ def lib():
 
 class OrcOpts(C.Structure):
     _fields_ = [("gamma", C.c_double), ("ruiz_its", C.c_int), ("cg_tol", C.c_double),
-                ("cg_itmax", C.c_int), ("pivot_tol", C.c_double), ("norm_tol", C.c_double)]
+                ("cg_itmax", C.c_int), ("pivot_tol", C.c_double), ("norm_tol", C.c_double),
+                ("chol_kind", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double)]
 
 
 class OrcStats(C.Structure):
     _fields_ = [("resid2", C.c_double), ("resid4", C.c_double), ("cg_iters", C.c_int),
                 ("cg_status", C.c_int), ("pivot", C.c_int), ("nnz_htil", C.c_int),
-                ("nnz_hgam", C.c_int), ("nnz_l", C.c_longlong), ("t_symbolic", C.c_double),
-                ("t_numeric", C.c_double)]
+                ("nnz_hgam", C.c_int), ("nnz_l", C.c_longlong), ("resid4_first", C.c_double),
+                ("refine_steps", C.c_int), ("t_symbolic", C.c_double), ("t_numeric", C.c_double)]
 
 
 class OrcIntermediates(C.Structure):
@@ -230,9 +233,10 @@ def perm_maps(rows, cols, perm, nrows=None):
 class Chol:
     """Oracle sparse Cholesky of the lower triangle of a CSR matrix."""
 
-    def __init__(self, A: Csr):
+    def __init__(self, A: Csr, kind=0):
+        """kind 0: up-looking; kind 1: supernodal left-looking (OpenMP)."""
         self.A = A
-        self.h = lib().orc_chol_create(A.n, A.rp, A.ci)
+        self.h = lib().orc_chol_create_kind(A.n, A.rp, A.ci, kind)
 
     def factor(self, tol=1e-12, vals=None):
         return lib().orc_chol_factor(self.h, self.A.v if vals is None else vals, tol)
@@ -266,13 +270,21 @@ def schur_cg(J: Csr, Jt: Csr, chol: Chol, b, x0=None, tol=1e-12, itmax=100):
     return x, st, it.value, fn.value
 
 
+def threads():
+    """OpenMP threads the oracle's parallel loops use."""
+    return int(lib().orc_threads())
+
+
 def execute(H: Csr, Ds_vals, Jc: Csr, Jd: Csr, rx, rs, ry, ryd, gamma, perm=None, y0=None,
             ruiz_its=2, cg_tol=1e-12, cg_itmax=100, pivot_tol=1e-12, norm_tol=1e-2,
-            intermediates=False):
-    """Oracle execute(); returns dict with dx, ds, dyc, dyd, stats (+ intermediates)."""
+            intermediates=False, chol_kind=0, refine_max=3, refine_tol=1e-9):
+    """Oracle execute(); returns dict with dx, ds, dyc, dyd, stats (+ intermediates).
+    chol_kind 1 selects the supernodal Cholesky (same factor to rounding);
+    refine_max 0 stops after the reference's single pass."""
     L = lib()
     n, m, d = H.n, Jc.n, len(Ds_vals)
-    o = OrcOpts(gamma, ruiz_its, cg_tol, cg_itmax, pivot_tol, norm_tol)
+    o = OrcOpts(gamma, ruiz_its, cg_tol, cg_itmax, pivot_tol, norm_tol, chol_kind, refine_max,
+                refine_tol)
     st = OrcStats()
     dx, dsv, dyc, dyd = np.zeros(n), np.zeros(d), np.zeros(m), np.zeros(d)
     pm = None if perm is None else np.ascontiguousarray(perm, np.int32)

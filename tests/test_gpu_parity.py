@@ -325,3 +325,29 @@ def test_indefinite_hgamma_returns_failure():
     bind(s, Hc, sy.Ds.v * 0 + 1e-8, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd)
     assert s.execute() == 1
     assert s.stats()["pivot"] >= 0
+
+
+def test_iterative_refinement_gpu():
+    """Ds spread 1e-4..1e8, gamma 1e5: the single pass (refine_max=0, the
+    reference) leaves resid4 above 1e-9, as the oracle's single pass does;
+    with refinement resid4 <= 1e-10 and the solution matches the oracle's
+    refined one."""
+    g = GridPattern(50, 65, seed=3, gamma=1e5, ds_range=(1e-4, 1e8))
+    sy = g.system()
+    Hc, Jc, Jd = to_csr(sy.H), to_csr(sy.Jc), to_csr(sy.Jd)
+    s = H.HykktSolver(sy.gamma)
+    bind(s, Hc, sy.Ds.v, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd)
+    s.options(refine_max=0)
+    assert s.execute() == 0
+    st0 = s.stats()
+    assert st0["refine_steps"] == 0 and st0["resid4"] > 1e-9
+    s.options(refine_max=3)
+    assert s.execute() == 0
+    st = s.stats()
+    assert st["refine_steps"] >= 1 and st["resid4"] <= 1e-10
+    assert st["resid4_first"] == st0["resid4"]
+    ref = O.execute(Hc, sy.Ds.v, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd, sy.gamma,
+                    perm=s.permutation())
+    x = np.concatenate(s.solution())
+    xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
+    assert rel(x, xo) < 1e-8

@@ -87,12 +87,14 @@ def test_perm_maps_closed_form():
         assert np.all(np.diff(mp["rc"][1][mp["rc"][0][i]:mp["rc"][0][i + 1]]) > 0)
 
 
-def test_cholesky_kat(mats):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_cholesky_kat(mats, kind):
     """examples/cuSolver_driver_cholesky.cpp:32-75: lower-triangle file read as
-    general, x[0], x[6] within 3e-2; plus the exact dense solution."""
+    general, x[0], x[6] within 3e-2; plus the exact dense solution.  Both
+    forms of the oracle factorization (up-looking, supernodal)."""
     H = O.read_mm(mats("a200_chol/H_matrix_ACTIVSg200_AC_00.mtx"), 3, False)
     b = O.read_rhs(mats("a200_chol/rhs_chol_ACTIVSg200_AC_00.mtx"))
-    ch = O.Chol(H)
+    ch = O.Chol(H, kind)
     assert ch.factor(1e-12) == -1
     x = ch.solve(b)
     assert abs(x[0] + 97.663191790329750) < 3e-2
@@ -102,6 +104,77 @@ def test_cholesky_kat(mats):
     xd = np.linalg.solve(Hd, b)
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
     assert np.linalg.norm(Hd @ x - b) / np.linalg.norm(b) < 1e-12
+
+
+def _spd(n, dens, seed, shift=2.0):
+    import scipy.sparse as sp
+    A = sp.random(n, n, density=dens, random_state=seed, format="csr")
+    A = (A + A.T + sp.eye(n) * (0.1 * n * dens * 10 + shift)).tocsr()
+    A.sort_indices()
+    return A, O.Csr(n, n, A.indptr.astype(np.int32), A.indices.astype(np.int32),
+                    A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("n,dens,seed", [(1, 1.0, 0), (7, 0.4, 1), (60, 0.08, 2),
+                                         (400, 0.015, 3), (4000, 0.001, 4)])
+def test_cholesky_supernodal_matches_uplooking(n, dens, seed):
+    """The supernodal form factors the same matrix in another order: same
+    nnz(L) and flop count (pattern only), solutions equal to rounding, and a
+    small residual against the matrix itself (scipy)."""
+    A, C = _spd(n, dens, seed)
+    up, snd = O.Chol(C, 0), O.Chol(C, 1)
+    assert up.factor() == -1 and snd.factor() == -1
+    assert up.nnz == snd.nnz and up.flops == snd.flops
+    b = np.random.default_rng(seed).standard_normal(n)
+    x0, x1 = up.solve(b), snd.solve(b)
+    assert np.linalg.norm(x1 - x0) <= 1e-12 * np.linalg.norm(x0)
+    assert np.linalg.norm(A @ x1 - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_cholesky_supernodal_dense_and_arrow():
+    """Dense SPD (one supernode) and a block-arrow matrix (several supernodes
+    feeding a dense border) against numpy's dense Cholesky solve."""
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((150, 150))
+    D = M @ M.T + 150 * np.eye(150)
+    blocks = [D]
+    nb, bs, border = 3, 90, 40
+    n = nb * bs + border
+    Aa = np.zeros((n, n))
+    for k in range(nb):
+        Q = rng.standard_normal((bs, bs))
+        Aa[k * bs:(k + 1) * bs, k * bs:(k + 1) * bs] = Q @ Q.T + bs * np.eye(bs)
+        Bk = rng.standard_normal((border, bs))
+        Aa[nb * bs:, k * bs:(k + 1) * bs] = Bk
+        Aa[k * bs:(k + 1) * bs, nb * bs:] = Bk.T
+    Aa[nb * bs:, nb * bs:] = 10 * n * np.eye(border)
+    blocks.append(Aa)
+    for Dm in blocks:
+        import scipy.sparse as sp
+        m = Dm.shape[0]
+        nzr, nzc = np.nonzero(Dm)
+        S = sp.csr_matrix((Dm[nzr, nzc], (nzr, nzc)), shape=(m, m))
+        S.sort_indices()
+        C = O.Csr(m, m, S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+        ch = O.Chol(C, 1)
+        assert ch.factor() == -1
+        b = rng.standard_normal(m)
+        x = ch.solve(b)
+        xd = np.linalg.solve(Dm, b)
+        assert np.linalg.norm(x - xd) <= 1e-11 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_cholesky_pivot_failure_index(kind):
+    """One negative diagonal entry: both forms report that row."""
+    n = 30
+    import scipy.sparse as sp
+    A = sp.diags([np.full(n - 1, 0.1), np.full(n, 4.0), np.full(n - 1, 0.1)], [-1, 0, 1]).tolil()
+    A[17, 17] = -1.0
+    A = A.tocsr()
+    A.sort_indices()
+    C = O.Csr(n, n, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data)
+    assert O.Chol(C, kind).factor() == 17
 
 
 def test_schur_cg_kat(mats):
@@ -282,3 +355,32 @@ def test_execute_fixtures(mats, kind, step, nnz_htil, nnz_hgam):
     assert st["nnz_htil"] == nnz_htil and st["nnz_hgam"] == nnz_hgam
     assert out["status"] == 0 and st["pivot"] == -1 and st["cg_status"] == 0
     assert st["resid4"] <= 1e-8
+    # the supernodal form of the factorization gives the same solve
+    sn = O.execute(H, s["Ds"].v, Jc, s["Jd"], s["rx"], s["rs"], s["ry"], s["ryd"], 1e4,
+                   perm=perm, chol_kind=1)
+    assert sn["status"] == 0 and sn["stats"]["resid4"] <= 1e-8
+    for k in ("dx", "ds", "dyc", "dyd"):
+        assert np.linalg.norm(sn[k] - out[k]) <= 1e-7 * max(np.linalg.norm(out[k]), 1e-300)
+
+
+def test_execute_iterative_refinement():
+    """Ds spread 1e-4..1e8 with gamma 1e5: the reference's single pass leaves
+    resid4 ~ 1e-7; refinement on the 4x4 system brings it below 1e-10 and the
+    solution onto the dense LU solution.  refine_max=0 is the single pass."""
+    g = GridPattern(50, 65, seed=3, gamma=1e5, ds_range=(1e-4, 1e8))
+    s = g.system()
+    c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
+    H, Jc, Jd = c(s.H), c(s.Jc), c(s.Jd)
+    one = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma, refine_max=0)
+    ref = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma)
+    assert one["stats"]["refine_steps"] == 0
+    assert one["stats"]["resid4"] == one["stats"]["resid4_first"] > 1e-9
+    assert ref["stats"]["resid4_first"] == one["stats"]["resid4"]
+    assert ref["stats"]["refine_steps"] >= 1 and ref["stats"]["resid4"] <= 1e-10
+    K = dense_kkt(H, s.Ds.v, Jc, Jd)
+    rhs = np.concatenate([s.rx, s.rs, s.ry, s.ryd])
+    sol = np.linalg.solve(K, rhs)
+    mine = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
+    assert np.linalg.norm(K @ mine - rhs) / np.linalg.norm(rhs) <= 1e-10
+    single = np.concatenate([one["dx"], one["ds"], one["dyc"], one["dyd"]])
+    assert np.linalg.norm(mine - sol) < np.linalg.norm(single - sol)

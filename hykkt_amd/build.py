@@ -15,6 +15,8 @@ ROOT = os.path.dirname(PKG)
 SRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhykkt.so")
+BIN = os.path.join(PKG, "bin")
+DRIVERS = ["solver_driver", "solver_emulator"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
@@ -64,6 +66,14 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the reference's example drivers, rebuilt on the C++ HykktSolver header
+    os.makedirs(BIN, exist_ok=True)
+    for name in DRIVERS:
+        out = os.path.join(BIN, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+              os.path.join(PKG, "drivers", name + ".cpp"), "-o", out + ".tmp", f"-L{PKG}",
+              "-lhykkt", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib"])
+        os.replace(out + ".tmp", out)
     return LIB
 
 

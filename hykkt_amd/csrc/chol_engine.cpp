@@ -211,14 +211,36 @@ void CholEngine::build_tasks()
 void CholEngine::build_solve()
 {
   const int ns = pl_.ns;
-  std::vector<int> fwd, bwd, pblk(ns + 1, 0);
+  // top of the tree: supernodes wider than one block or with fronts taller
+  // than TOP_F rows, and all their ancestors; the rest is a forest closed
+  // under descendants
+  constexpr int TOP_F = 512;
+  std::vector<char> top(ns, 0);
+  for (int s = 0; s < ns; s++)
+    if (pl_.nc(s) > NB || pl_.nc(s) + pl_.nr(s) > TOP_F) top[s] = 1;
+  for (int s = 0; s < ns; s++)
+    if (top[s])
+      for (int p = pl_.sn_parent[s]; p >= 0 && !top[p]; p = pl_.sn_parent[p]) top[p] = 1;
+  std::vector<int> fwd, bwd, fwd_top, bwd_top, pblk(ns + 1, 0);
   for (int s = 0; s < ns; s++) pblk[s + 1] = pblk[s] + cdiv(pl_.nc(s), NB);
+  solve_lower_[0] = solve_lower_[1] = 0.0;
+  solve_n_[0] = solve_n_[1] = 0;
   for (int s = 0; s < ns; s++) {   // postorder: children first
-    for (int b = 0; b < cdiv(pl_.nc(s), NB); b++) fwd.insert(fwd.end(), {s, 0, b});
-    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++) fwd.insert(fwd.end(), {s, 1, u});
+    std::vector<int>& f = top[s] ? fwd_top : fwd;
+    for (int b = 0; b < cdiv(pl_.nc(s), NB); b++) f.insert(f.end(), {s, 0, b});
+    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++) f.insert(f.end(), {s, 1, u});
+    const double nc = pl_.nc(s), nr = pl_.nr(s);
+    solve_lower_[top[s]] += nc * (nc + 1) / 2 + nr * nc;
+    solve_n_[top[s]] += pl_.nc(s);
   }
-  for (int s = ns - 1; s >= 0; s--)   // parents first
-    for (int b = cdiv(pl_.nc(s), NB) - 1; b >= 0; b--) bwd.insert(bwd.end(), {s, b});
+  std::vector<int64_t> bpo(ns + 1, 0);
+  for (int s = ns - 1; s >= 0; s--) {   // parents first: row chunks, then pivot blocks
+    std::vector<int>& b = top[s] ? bwd_top : bwd;
+    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++) b.insert(b.end(), {s, 0, u});
+    for (int k = cdiv(pl_.nc(s), NB) - 1; k >= 0; k--) b.insert(b.end(), {s, 1, k});
+  }
+  for (int s = 0; s < ns; s++)
+    bpo[s + 1] = bpo[s] + static_cast<int64_t>(cdiv(pl_.nr(s), NB)) * pl_.nc(s);
   // pull lists: front row of a parent <- entries of its children's update vectors
   std::vector<int64_t> foff(ns + 1, 0);
   for (int s = 0; s < ns; s++) foff[s + 1] = foff[s] + pl_.nc(s) + pl_.nr(s);
@@ -245,27 +267,44 @@ void CholEngine::build_solve()
     if (v.empty()) v.push_back(0);
     b.upload(v);
   };
+  sd_.n_fwd = static_cast<int>(fwd.size() / 3);
+  sd_.n_bwd = static_cast<int>(bwd.size() / 3);
+  sd_.n_fwd_top = static_cast<int>(fwd_top.size() / 3);
+  sd_.n_bwd_top = static_cast<int>(bwd_top.size() / 3);
   upi(fwd_items_, fwd);
   upi(bwd_items_, bwd);
+  upi(fwd_top_items_, fwd_top);
+  upi(bwd_top_items_, bwd_top);
   upi(pblk_off_, pblk);
   upi(pull_ptr_, cnt);
   upi(pull_src_, src);
   front_off_.upload(foff);
-  const int nflags = 2 * pblk[ns] + 2 * ns + 1;
+  bpart_off_.upload(bpo);
+  bpart_.alloc(static_cast<size_t>(std::max<int64_t>(bpo[ns], 1)));
+  const int npb = pblk[ns];
+  const int nflags = 2 * npb + 3 * ns + 1;
   flags_.alloc(nflags);
   sd_.fwd_items = fwd_items_.p;
-  sd_.n_fwd = static_cast<int>(fwd.size() / 3);
   sd_.bwd_items = bwd_items_.p;
-  sd_.n_bwd = static_cast<int>(bwd.size() / 2);
+  sd_.fwd_items_top = fwd_top_items_.p;
+  sd_.bwd_items_top = bwd_top_items_.p;
   sd_.pblk_off = pblk_off_.p;
-  sd_.sn_parent = sn_parent_.p;
   sd_.uvec_off = uvec_off_.p;
   sd_.pull_ptr = pull_ptr_.p;
   sd_.pull_src = pull_src_.p;
   sd_.front_off = front_off_.p;
   sd_.ubuf = ubuf_.p;
+  sd_.bpart = bpart_.p;
+  sd_.bpart_off = bpart_off_.p;
   sd_.flags = flags_.p;
-  sd_.n_pblk = pblk[ns];
+  sd_.n_flags = nflags;
+  sd_.fflag = flags_.p;
+  sd_.bflag = flags_.p + npb;
+  sd_.fcount = flags_.p + 2 * npb;
+  sd_.bcount = flags_.p + 2 * npb + ns;
+  sd_.ccount = flags_.p + 2 * npb + 2 * ns;
+  sd_.err = flags_.p + 2 * npb + 3 * ns;
+  sd_.n_pblk = npb;
   dev::fill_i(flags_.p, 0, nflags, st_);
 }
 
@@ -337,28 +376,32 @@ int CholEngine::pivot()
 int CholEngine::solve_error()
 {
   int h = 0;
-  const int idx = 2 * sd_.n_pblk + 2 * pl_.ns;
-  HIPCHK(hipMemcpyAsync(&h, flags_.p + idx, sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(&h, sd_.err, sizeof(int), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
+  if (h) dev::fill_i(sd_.err, 0, 1, st_);
   return h;
 }
 
 void CholEngine::solve(double* x)
 {
-  dev::solve_reset(sd_, pl_.ns, st_);
+  dev::solve_reset(sd_, st_);
   Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
-  // Algorithmic: each sweep reads the true factor once, 2 flops per entry,
-  // and reads + writes the vector.
-  const double fl = 2.0 * static_cast<double>(pl_.nnz_l_true);
-  const double by = 8.0 * (static_cast<double>(pl_.nnz_l_true) + 2.0 * pl_.n);
-  if (pf) pf->begin(st_);
-  dev::solve_forward(cd_, sd_, x, st_);
-  if (pf) {
-    pf->end(PK_SOLVE_FWD, st_, fl, by);
-    pf->begin(st_);
-  }
-  dev::solve_backward(cd_, sd_, x, st_);
-  if (pf) pf->end(PK_SOLVE_BWD, st_, fl, by);
+  // Algorithmic, per part of the tree: each sweep reads that part's factor
+  // entries (lower trapezoids of its panels) once, 2 flops per entry, and
+  // reads + writes its vector entries.
+  auto sweep = [&](int kind, int part, bool fwd) {
+    if ((part == 0 ? (fwd ? sd_.n_fwd : sd_.n_bwd) : (fwd ? sd_.n_fwd_top : sd_.n_bwd_top)) <= 0)
+      return;
+    if (pf) pf->begin(st_);
+    if (fwd) dev::solve_forward(cd_, sd_, x, part, st_);
+    else dev::solve_backward(cd_, sd_, x, part, st_);
+    if (pf)
+      pf->end(kind, st_, 2.0 * solve_lower_[part], 8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]));
+  };
+  sweep(PK_SOLVE_FWD_FOREST, 0, true);
+  sweep(PK_SOLVE_FWD_TOP, 1, true);
+  sweep(PK_SOLVE_BWD_TOP, 1, false);
+  sweep(PK_SOLVE_BWD_FOREST, 0, false);
 }
 
 }  // namespace hykkt

@@ -75,7 +75,8 @@ class CholEngine {
   // First failing pivot (internal index) or -1; synchronizes the stream.
   int pivot();
   // Nonzero if a triangular-solve sweep hit its wait bound since the last
-  // call (synchronizes the stream, clears the flag).
+  // call (synchronizes the stream, clears the flag; solves in between leave
+  // it set).
   int solve_error();
   // Solve L L^T x = b in place, x in internal (postordered) order.
   void solve(double* x);
@@ -116,6 +117,11 @@ class CholEngine {
   DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
   // solve items
   DevBuf<int> fwd_items_, bwd_items_, pblk_off_, pull_ptr_, pull_src_, flags_;
+  DevBuf<int64_t> bpart_off_;
+  DevBuf<double> bpart_;
+  DevBuf<int> fwd_top_items_, bwd_top_items_;
+  double solve_lower_[2] = {0.0, 0.0};   // stored lower-trapezoid entries: forest, top
+  int64_t solve_n_[2] = {0, 0};          // columns: forest, top
 };
 
 // The Schur-complement operator S = Jh A^{-1} Jh^T with A factored by `chol`

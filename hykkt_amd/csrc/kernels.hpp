@@ -133,25 +133,41 @@ void dense_nt(const CholDev& c, int ntasks, const int* task_sn, const int* task_
 // Single-launch supernodal solves over the whole tree (L y = b, then L^T x = y,
 // in place on x, internal order).  Work items are 128-row blocks of the
 // fronts, dispatched in a dependency-respecting order; each item waits for the
-// items it depends on through flags in `sync` (reset by solve_reset).
+// items it depends on through flags (reset by solve_reset).
 struct SolveDev {
+  // Items of the small-supernode forest (fwd_items, bwd_items) and of the top
+  // of the tree (*_top) are separate launches: forward forest then top,
+  // backward top then forest.
   const int* fwd_items;     // 3 ints per item: supernode, kind (0 pivot, 1 update), block
   int n_fwd;
-  const int* bwd_items;     // 2 ints per item: supernode, block
+  const int* bwd_items;     // 3 ints per item: supernode, kind (0 row chunk, 1 pivot), index
   int n_bwd;
+  const int* fwd_items_top;
+  int n_fwd_top;
+  const int* bwd_items_top;
+  int n_bwd_top;
   const int* pblk_off;      // ns+1: global index of the first pivot block of s
-  const int* sn_parent;
   const int64_t* uvec_off;  // ns+1 offsets of the forward update vectors
   const int* pull_ptr;      // front-row pull lists (children contributions)
   const int* pull_src;
   const int64_t* front_off; // ns+1: offset of s's front rows in the pull_ptr index
   double* ubuf;
-  int* flags;               // n_pblk pivot-block flags, then ns fwd counters, ns bwd counters, err
+  double* bpart;            // backward chunk partials: chunk u of s at bpart_off[s] + u*nc
+  const int64_t* bpart_off;
+  int* flags;               // all flags below, one allocation, err last
+  int n_flags;
+  int* fflag;               // n_pblk: forward pivot block done
+  int* bflag;               // n_pblk: backward pivot block done
+  int* fcount;              // ns: forward update items done
+  int* bcount;              // ns: backward pivot items done
+  int* ccount;              // ns: backward chunk items done
+  int* err;                 // 1: a wait hit its bound
   int n_pblk;
 };
-void solve_reset(const SolveDev& sd, int ns, hipStream_t st);
-void solve_forward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st);
-void solve_backward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st);
+void solve_reset(const SolveDev& sd, hipStream_t st);
+// part 0: the forest items, part 1: the top items
+void solve_forward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st);
+void solve_backward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st);
 
 }  // namespace dev
 }  // namespace hykkt

@@ -1,25 +1,48 @@
 // kernels_trsv.hip -- supernodal triangular solves L y = b and L^T x = y as
 // two single-launch sweeps over the whole elimination tree (gfx950).
 //
-// Work items are 128-row blocks of the supernodal fronts.  In the forward
-// sweep a pivot block of supernode s (rows of its own columns) waits for the
-// update vectors of s's children and for the earlier pivot blocks of s,
-// accumulates L(blk, j) * y_j as those become ready, and finishes with the
-// inverted diagonal block: y_blk = Dinv_blk (v - acc).  An update block
-// (rows of s below its columns) accumulates the same products into s's
-// update vector, which the parent pulls.  The backward sweep runs the
-// transpose from the root down.
+// Work items are 128-row blocks of the supernodal fronts, 1024 threads each:
+// thread (row = tid % 128, slice q = tid / 128) owns one row (forward) or one
+// column (backward) of a 128 x 128 block and a 16-wide slice of the other
+// index, so a whole block and an inverted diagonal block fit in registers.
+//
+// Forward sweep, per supernode s in postorder:
+//   pivot item (s, b): waits for s's children's update vectors and for the
+//     earlier pivot blocks of s, accumulates L(b, j) y_j as each y_j becomes
+//     ready, and finishes with the inverted diagonal block:
+//     y_b = Dinv_b (v - acc);
+//   update item (s, u): the same products for 128 rows below s's columns,
+//     written to s's update vector, which the parent pulls.
+// Backward sweep, per supernode from the root down:
+//   chunk item (s, u): once s's parent is done, all of x(rows of s) is final;
+//     the item computes L(rows of chunk u, :)^T x(rows) for every column of s
+//     into a partial-sum buffer (chunks run in parallel);
+//   pivot item (s, b): sums the chunk partials in a fixed order, then
+//     L(j, b)^T x_j over the later pivot blocks j > b as they finish, and
+//     x_b = Dinv_b^T (t - acc).
+// The L block of the next step, and the inverted diagonal block before the
+// last wait, are loaded into registers before the item waits, so a link of
+// the dependency chain costs the flag round trip and arithmetic only.
 //
 // Items are dispatched in an order where every item depends only on items
-// with a smaller index.  The GPU dispatches the workgroups of each XCD in
+// with a smaller index.  Each XCD dispatches its share of the workgroups in
 // increasing index order, so the smallest unfinished item is always resident
-// and never blocked: the sweep cannot deadlock.  Every wait is additionally
+// and runnable: the sweep cannot deadlock.  Every wait is additionally
 // bounded (SPIN_LIMIT); on expiry the item sets an error flag the host
 // reports, and carries on, so the kernel always terminates.
 //
-// Cross-workgroup visibility: the producer makes its stores visible with a
-// device-scope fence before publishing its flag with a device-scope atomic;
-// the consumer fences after observing the flag.
+// Cross-workgroup visibility.  The eight XCDs of the chip have separate L2
+// caches, so a device-scope release/acquire fence writes back / invalidates
+// a whole L2, which would cost every link of the chain microseconds and evict
+// the factor blocks other items are streaming.  Instead, every value passed
+// between items (x blocks, update vectors, chunk partials) is written and read
+// with device-scope relaxed atomic stores and loads, which go to the
+// device-coherent level, and the producer waits for its stores to complete
+// (s_waitcnt: on gfx9-family GPUs the vector-memory counter covers stores)
+// before the workgroup barrier that precedes the flag increment.  A consumer
+// observes the flag with a device-scope atomic load, passes a barrier, and
+// only then issues its device-scope loads of the data.  Factor values are
+// plain loads: they were written by earlier kernels.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -29,8 +52,13 @@ namespace dev {
 
 namespace {
 
-constexpr int SB = 128;                   // item rows / diagonal block size
-constexpr int SPIN_LIMIT = 1 << 22;   // polls of ~100 cycles: ~0.2 s per wait
+constexpr int SB = 128;                  // item rows / diagonal block size
+// Items of the top of the tree (supernodes wider than one block, fronts
+// taller than TOP_F rows, and their ancestors) use NT_TOP threads with
+// register prefetch; the small-supernode forest below them uses NT_FOREST.
+constexpr int NT_TOP = 1024;
+constexpr int NT_FOREST = 256;
+constexpr int SPIN_LIMIT = 1 << 22;      // polls of ~100 cycles: ~0.2 s per wait
 
 struct SnInfo {
   int a, nc, nr, f, np, nu;
@@ -51,7 +79,7 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 }
 
 // Block until *p >= target: thread 0 spins on a device-scope atomic load,
-// then every thread acquires at device scope.  Gives up (and sets *err) after
+// then the workgroup passes a barrier.  Gives up (and sets *err) after
 // SPIN_LIMIT polls, or at once if another item already gave up.
 __device__ void wait_geq(int* p, int target, int* err)
 {
@@ -66,14 +94,14 @@ __device__ void wait_geq(int* p, int target, int* err)
     }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Publish: every thread releases its stores at device scope, then one thread
-// increments the flag.
+// Publish: every thread waits until its (device-scope atomic) stores have
+// completed, the workgroup passes a barrier, then one thread increments the
+// flag(s).
 __device__ void signal_add(int* p, int* p2 = nullptr)
 {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -81,10 +109,45 @@ __device__ void signal_add(int* p, int* p2 = nullptr)
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+// Fixed-order sum of the NQ slice partials of one row (column).
+template <int NQ>
+__device__ __forceinline__ double sum_slices(const double (&part)[NQ][SB], int i)
 {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int k = 0; k < NQ; k += 2) {
+    a += part[k][i];
+    b += part[k + 1][i];
+  }
+  return a + b;
+}
+
+// Buffer loads: a uniform base in a buffer descriptor (SGPRs), a per-lane
+// byte offset and a uniform byte offset, so a stretch of strided loads costs
+// scalar arithmetic and one offset register instead of one address pair per
+// load.  Word 3 of the descriptor is the gfx9-family raw-buffer setting.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int RSRC_W3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t view(const double* base, uint32_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, bytes, RSRC_W3);
+}
+
+__device__ __forceinline__ double ld(rsrc_t r, uint32_t voff, uint32_t soff)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Values exchanged between items: device-scope relaxed atomics.
+__device__ __forceinline__ double xload(const double* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void xstore(double* p, double v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_solve_reset(int* flags, int n)
@@ -93,205 +156,291 @@ __global__ void k_solve_reset(int* flags, int n)
   if (i < n) flags[i] = 0;
 }
 
-// ---------------------------------------------------------------- forward
-__global__ void __launch_bounds__(256) k_solve_fwd(CholDev c, SolveDev sd, double* __restrict__ x)
+// A slice of QW values, value k at byte offset vo + so0 + k*sstride of the
+// view rs, for k < kvalid (uniform); other values and dead lanes read 0.
+template <int QW>
+__device__ __forceinline__ void slice_load(double (&v)[QW], rsrc_t rs, uint32_t vo, uint32_t so0,
+                                           uint32_t sstride, int kvalid, bool live)
 {
+  const uint32_t vl = live ? vo : 0u;
+#pragma unroll
+  for (int k = 0; k < QW; k++) {
+    const bool in = k < kvalid;
+    const double t = ld(rs, vl, in ? so0 + static_cast<uint32_t>(k) * sstride : 0u);
+    v[k] = (live && in) ? t : 0.0;
+  }
+}
+
+template <int QW>
+__device__ __forceinline__ double slice_dot(const double (&v)[QW], const double* vec)
+{
+  double a = 0.0;
+#pragma unroll
+  for (int k = 0; k < QW; k++) a += v[k] * vec[k];
+  return a;
+}
+
+// The same dot product with the slice streamed through registers 16 values
+// at a time (no prefetch; few registers).
+template <int QW>
+__device__ __forceinline__ double slice_dot_stream(rsrc_t rs, uint32_t vo, uint32_t so0,
+                                                   uint32_t sstride, int kvalid, bool live,
+                                                   const double* vec)
+{
+  constexpr int CH = QW < 16 ? QW : 16;
+  double a = 0.0;
+#pragma unroll
+  for (int k0 = 0; k0 < QW; k0 += CH) {
+    double t[CH];
+    slice_load<CH>(t, rs, vo, so0 + static_cast<uint32_t>(k0) * sstride, sstride, kvalid - k0,
+                   live);
+#pragma unroll
+    for (int k = 0; k < CH; k++) a += t[k] * vec[k0 + k];
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------- forward
+// NT threads per item; PF: the L slice of the next step and the inverted
+// diagonal block are prefetched into registers before the waits (top of the
+// tree); otherwise slices are streamed after each wait (small supernodes,
+// high occupancy).
+template <int NT, bool PF>
+__global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
+                                                  const int* __restrict__ items,
+                                                  double* __restrict__ x)
+{
+  constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double yv[SB];
-  __shared__ double part[SB];
-  const int s = sd.fwd_items[3 * blockIdx.x];
-  const int kind = sd.fwd_items[3 * blockIdx.x + 1];
-  const int blk = sd.fwd_items[3 * blockIdx.x + 2];
+  __shared__ double part[NQ][SB];
+  const int s = items[3 * blockIdx.x];
+  const int kind = items[3 * blockIdx.x + 1];
+  const int blk = items[3 * blockIdx.x + 2];
   const SnInfo si = sninfo(c, s);
-  int* fflag = sd.flags;                              // forward pivot-block flags
-  int* fcount = sd.flags + 2 * sd.n_pblk;             // forward update-block counters
-  int* err = sd.flags + 2 * sd.n_pblk + 2 * c.ns;
   const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
   const int rend = (kind == 0) ? min(si.nc, r0 + SB) : min(si.f, r0 + SB);
   const int nrows = rend - r0;
-  const int row = threadIdx.x & (SB - 1), half = threadIdx.x >> 7;
+  const int row = threadIdx.x & (SB - 1);
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // whole waves per slice
+  const bool live = row < nrows;
   const int r = r0 + row;
+  const int jmax = (kind == 0) ? blk : si.np;
+  const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
+
+  // row r of column block j: L(r, j*128 + q*QW + k)
+  auto lview = [&](int j) {
+    return view(si.Ls + static_cast<int64_t>(j) * SB * si.f,
+                static_cast<uint32_t>(min(SB, si.nc - j * SB)) * f8);
+  };
+  auto lvalid = [&](int j) { return min(SB, si.nc - j * SB) - q * QW; };
+  const uint32_t vo = static_cast<uint32_t>(r) * 8u;
+  const uint32_t so = static_cast<uint32_t>(q * QW) * f8;
+  // row `row` of the inverted diagonal block (upper part stored as zeros)
+  const rsrc_t wv = view(c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB,
+                         SB * SB * 8u);
+  const uint32_t wvo = static_cast<uint32_t>(row) * 8u, wso = static_cast<uint32_t>(q * QW) * SB * 8u;
+
+  double l[PF ? QW : 1], w[PF ? QW : 1];
+  if constexpr (PF) {
+    if (jmax > 0) slice_load<QW>(l, lview(0), vo, so, f8, lvalid(0), live);
+    if (kind == 0 && jmax == 0) slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
+  }
 
   // children's update vectors must be complete before they are pulled
   for (int t = c.child_ptr[s]; t < c.child_ptr[s + 1]; t++) {
     const int ch = c.child_list[t];
     const int nu_ch = (c.rows_ptr[ch + 1] - c.rows_ptr[ch] + SB - 1) / SB;
-    wait_geq(fcount + ch, nu_ch, err);
+    wait_geq(sd.fcount + ch, nu_ch, sd.err);
   }
   double v = 0.0;
-  if (half == 0 && row < nrows) {
+  if (q == 0 && live) {
     v = (kind == 0) ? x[si.a + r] : 0.0;
     const int64_t fr = sd.front_off[s] + r;
-    for (int q = sd.pull_ptr[fr]; q < sd.pull_ptr[fr + 1]; q++) v += sd.ubuf[sd.pull_src[q]];
+    for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++) v += xload(sd.ubuf + sd.pull_src[p]);
   }
-  // v -= L(r, B_j) y_j for the earlier pivot blocks
-  const int jmax = (kind == 0) ? blk : si.np;
+  // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready
   for (int j = 0; j < jmax; j++) {
-    wait_geq(fflag + sd.pblk_off[s] + j, 1, err);
-    const int bj = min(SB, si.nc - j * SB);
-    if (threadIdx.x < SB) yv[threadIdx.x] = (threadIdx.x < bj) ? x[si.a + j * SB + threadIdx.x] : 0.0;
-    __syncthreads();
-    double tp = 0.0;
-    if (row < nrows) {
-      // 64 independent loads per thread (one column of L each), then the FMAs
-      const double* col = si.Ls + static_cast<int64_t>(j * SB + half * 64) * si.f + r;
-      const int cn = min(64, bj - half * 64);
-      double lv[64];
-#pragma unroll
-      for (int q = 0; q < 64; q++) lv[q] = (q < cn) ? col[static_cast<int64_t>(q) * si.f] : 0.0;
-#pragma unroll
-      for (int q = 0; q < 64; q++) tp += lv[q] * yv[half * 64 + q];
+    if constexpr (PF) {
+      if (kind == 0 && j == jmax - 1)   // in flight during the last wait
+        slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
     }
-    if (half == 1) part[row] = tp;
+    wait_geq(sd.fflag + sd.pblk_off[s] + j, 1, sd.err);
+    const int bj = min(SB, si.nc - j * SB);
+    if (threadIdx.x < SB) yv[threadIdx.x] = (threadIdx.x < bj) ? xload(x + si.a + j * SB + threadIdx.x) : 0.0;
     __syncthreads();
-    if (half == 0 && row < nrows) v -= tp + part[row];
+    double tp;
+    if constexpr (PF) tp = slice_dot<QW>(l, yv + q * QW);
+    else tp = slice_dot_stream<QW>(lview(j), vo, so, f8, lvalid(j), live, yv + q * QW);
+    part[q][row] = tp;
+    __syncthreads();
+    // the barrier keeps the next block's loads after the last use of l
+    if constexpr (PF) {
+      if (j + 1 < jmax) slice_load<QW>(l, lview(j + 1), vo, so, f8, lvalid(j + 1), live);
+    }
+    if (q == 0) v -= sum_slices<NQ>(part, row);
     __syncthreads();
   }
   if (kind == 0) {
-    // y = Dinv_blk * v  (Dinv lower triangular, column-major 128 x 128)
-    if (half == 0) part[row] = (row < nrows) ? v : 0.0;
+    // y = Dinv_blk v
+    if (q == 0) yv[row] = live ? v : 0.0;
     __syncthreads();
-    const double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB;
-    // y(row) = sum_{cc <= row} W(row, cc) v(cc): the two halves of the
-    // threads take the columns [0,64) and [64,128); loads issued 32 at a time
-    double y = 0.0;
-    if (row < nrows) {
-      const double* wr = W + row + static_cast<int64_t>(half * 64) * SB;
-#pragma unroll
-      for (int q0 = 0; q0 < 64; q0 += 32) {
-        double w[32];
-#pragma unroll
-        for (int q = 0; q < 32; q++) w[q] = wr[static_cast<int64_t>(q0 + q) * SB];
-#pragma unroll
-        for (int q = 0; q < 32; q++) y += w[q] * part[half * 64 + q0 + q];
-      }
-    }
+    double y;
+    if constexpr (PF) y = slice_dot<QW>(w, yv + q * QW);
+    else y = slice_dot_stream<QW>(wv, wvo, wso, SB * 8u, nrows - q * QW, live, yv + q * QW);
+    part[q][row] = y;
     __syncthreads();
-    if (half == 1) yv[row] = y;
-    __syncthreads();
-    if (half == 0 && row < nrows) x[si.a + r] = y + yv[row];
-    signal_add(fflag + sd.pblk_off[s] + blk);
+    if (q == 0 && live) xstore(x + si.a + r, sum_slices<NQ>(part, row));
+    signal_add(sd.fflag + sd.pblk_off[s] + blk);
   } else {
-    if (half == 0 && row < nrows) sd.ubuf[sd.uvec_off[s] + (r - si.nc)] = v;
-    signal_add(fcount + s);
+    if (q == 0 && live) xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v);
+    signal_add(sd.fcount + s);
   }
 }
 
 // ---------------------------------------------------------------- backward
-// Partial of sum_{k < nrow} L(row0 + k, c0 + cc) * xs[k] for this thread's
-// column cc = tid % 128 over its half of the rows.  Lp points at (row0, c0)
-// of a column-major panel with leading dimension f.  Each thread walks a
-// contiguous stretch of its column, 16 independent loads at a time (one
-// 128-byte line), so many lines are in flight per workgroup.
-__device__ __forceinline__ double gemv_t_cols(const double* Lp, int64_t f, int nrow, int ncol,
-                                              const double* xs)
+template <int NT, bool PF>
+__global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
+                                                  const int* __restrict__ items,
+                                                  double* __restrict__ x)
 {
-  const int cc = threadIdx.x & (SB - 1), half = threadIdx.x >> 7;
-  if (cc >= ncol) return 0.0;
-  const int h = ((nrow + 31) / 32) * 16;          // split on a 16-row boundary
-  const int rb = half ? min(h, nrow) : 0, re = half ? nrow : min(h, nrow);
-  const double* col = Lp + static_cast<int64_t>(cc) * f;
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  int k = rb;
-  for (; k + 16 <= re; k += 16) {
-    double v[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = col[k + q];
-#pragma unroll
-    for (int q = 0; q < 16; q++) a[q & 3] += v[q] * xs[k + q];
-  }
-  for (; k < re; k++) a[0] += col[k] * xs[k];
-  return (a[0] + a[1]) + (a[2] + a[3]);
-}
-
-__global__ void __launch_bounds__(256) k_solve_bwd(CholDev c, SolveDev sd, double* __restrict__ x)
-{
-  __shared__ double t[SB];
-  __shared__ double part[SB];
-  __shared__ double xs[1024];
-  const int s = sd.bwd_items[2 * blockIdx.x];
-  const int blk = sd.bwd_items[2 * blockIdx.x + 1];
+  constexpr int NQ = NT / SB, QW = SB / NQ;
+  __shared__ double xs[SB];
+  __shared__ double part[NQ][SB];
+  const int s = items[3 * blockIdx.x];
+  const int kind = items[3 * blockIdx.x + 1];
+  const int idx = items[3 * blockIdx.x + 2];
   const SnInfo si = sninfo(c, s);
-  int* bflag = sd.flags + sd.n_pblk;                  // backward pivot-block flags
-  int* bcount = sd.flags + 2 * sd.n_pblk + c.ns;      // backward per-supernode counters
-  int* err = sd.flags + 2 * sd.n_pblk + 2 * c.ns;
+  const int col = threadIdx.x & (SB - 1);
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // whole waves per slice
+  const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
+  // column `col` of column block starting at c0 (nb wide): L(row, c0 + col)
+  auto cview = [&](int c0, int nb) {
+    return view(si.Ls + static_cast<int64_t>(c0) * si.f, static_cast<uint32_t>(nb) * f8);
+  };
+  const uint32_t cvo = static_cast<uint32_t>(col) * f8;
+
+  if (kind == 0) {
+    // ---- chunk item: rows [nc + idx*128, ...) of s, every column block
+    const int rr0 = idx * SB;
+    const int nrows = min(SB, si.nr - rr0);
+    const int* rows = c.rows + c.rows_ptr[s] + rr0;
+    const uint32_t so = static_cast<uint32_t>(si.nc + rr0 + q * QW) * 8u;
+    double l[PF ? QW : 1];
+    if constexpr (PF) {
+      const int nb0 = min(SB, si.nc);
+      slice_load<QW>(l, cview(0, nb0), cvo, so, 8u, nrows - q * QW, col < nb0);
+    }
+    const int p = c.sn_parent[s];
+    if (p >= 0) {
+      const int np_p = (c.sn_first[p + 1] - c.sn_first[p] + SB - 1) / SB;
+      wait_geq(sd.bcount + p, np_p, sd.err);
+    }
+    if (threadIdx.x < SB) xs[threadIdx.x] = (threadIdx.x < nrows) ? xload(x + rows[threadIdx.x]) : 0.0;
+    __syncthreads();
+    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(idx) * si.nc;
+    for (int b = 0; b < si.np; b++) {
+      const int bb = min(SB, si.nc - b * SB);
+      double tp;
+      if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
+      else tp = slice_dot_stream<QW>(cview(b * SB, bb), cvo, so, 8u, nrows - q * QW, col < bb,
+                                     xs + q * QW);
+      part[q][col] = tp;
+      __syncthreads();
+      if constexpr (PF) {
+        if (b + 1 < si.np) {
+          const int nbn = min(SB, si.nc - (b + 1) * SB);
+          slice_load<QW>(l, cview((b + 1) * SB, nbn), cvo, so, 8u, nrows - q * QW, col < nbn);
+        }
+      }
+      if (q == 0 && col < bb) xstore(out + b * SB + col, sum_slices<NQ>(part, col));
+      __syncthreads();
+    }
+    signal_add(sd.ccount + s);
+    return;
+  }
+
+  // ---- pivot item: columns [c0, c0 + nb) of s
+  const int blk = idx;
   const int c0 = blk * SB;
   const int nb = min(SB, si.nc - c0);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cc = threadIdx.x & (SB - 1), half = threadIdx.x >> 7;
-  const int* rows = c.rows + c.rows_ptr[s];
+  const bool live = col < nb;
+  const rsrc_t lv = cview(c0, nb);
+  auto lso = [&](int j) { return static_cast<uint32_t>(j * SB + q * QW) * 8u; };
+  auto lvalid = [&](int j) { return min(SB, si.nc - j * SB) - q * QW; };
+  // Dinv_blk^T: x(col) = sum_{row >= col} W(row, col) t(row): column `col` of
+  // W (its upper part is stored as zeros)
+  const rsrc_t wv = view(c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB,
+                         SB * SB * 8u);
+  const uint32_t wvo = static_cast<uint32_t>(col) * SB * 8u, wso = static_cast<uint32_t>(q * QW) * 8u;
+  const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
+  double l[PF ? QW : 1], w[PF ? QW : 1];
+  if constexpr (PF) {
+    if (jtop > blk) slice_load<QW>(l, lv, cvo, lso(jtop), 8u, lvalid(jtop), live);
+    else slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);
+  }
 
-  const int p = c.sn_parent[s];
-  if (p >= 0) {
-    const int np_p = (c.sn_first[p + 1] - c.sn_first[p] + SB - 1) / SB;
-    wait_geq(bcount + p, np_p, err);
+  if (si.nu > 0) wait_geq(sd.ccount + s, si.nu, sd.err);
+  double t = 0.0;
+  if (q == 0 && live) {
+    t = x[si.a + c0 + col];
+    const double* bp = sd.bpart + sd.bpart_off[s] + c0 + col;
+    for (int u = 0; u < si.nu; u++) t -= xload(bp + static_cast<int64_t>(u) * si.nc);
   }
-  if (threadIdx.x < SB) t[threadIdx.x] = (threadIdx.x < nb) ? x[si.a + c0 + threadIdx.x] : 0.0;
-  double acc = 0.0;
-  // L21(:, block)^T x(rows), x(rows) gathered through LDS 1024 at a time
-  for (int k0 = 0; k0 < si.nr; k0 += 1024) {
-    const int kn = min(1024, si.nr - k0);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kn; k += blockDim.x) xs[k] = x[rows[k0 + k]];
-    __syncthreads();
-    acc += gemv_t_cols(si.Ls + static_cast<int64_t>(c0) * si.f + si.nc + k0, si.f, kn, nb, xs);
-  }
-  // L(B_j, block)^T x_j for the later pivot blocks of s
-  for (int j = si.np - 1; j > blk; j--) {
-    wait_geq(bflag + sd.pblk_off[s] + j, 1, err);
-    const int bj = min(SB, si.nc - j * SB);
-    for (int k = threadIdx.x; k < bj; k += blockDim.x) xs[k] = x[si.a + j * SB + k];
-    __syncthreads();
-    acc += gemv_t_cols(si.Ls + static_cast<int64_t>(c0) * si.f + j * SB, si.f, bj, nb, xs);
-    __syncthreads();
-  }
-  if (half == 1) part[cc] = acc;
-  __syncthreads();
-  if (half == 0) t[cc] -= acc + part[cc];
-  __syncthreads();
-  // x_block = Dinv^T t: column q of W (contiguous) dotted with t, two threads
-  // per column, 16 independent loads at a time
-  const double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB;
-  {
-    const double* wc = W + static_cast<int64_t>(cc) * SB + half * 64;
-    double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-    for (int q0 = 0; q0 < 64; q0 += 16) {
-      double w[16];
-#pragma unroll
-      for (int q = 0; q < 16; q++) w[q] = wc[q0 + q];
-#pragma unroll
-      for (int q = 0; q < 16; q += 2) {
-        a0 += w[q] * t[half * 64 + q0 + q];
-        a1 += w[q + 1] * t[half * 64 + q0 + q + 1];
-      }
+  for (int j = jtop; j > blk; j--) {
+    if constexpr (PF) {
+      if (j == blk + 1) slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);   // last wait
     }
-    if (half == 1) part[cc] = a0 + a1;
+    wait_geq(sd.bflag + sd.pblk_off[s] + j, 1, sd.err);
+    const int bj = min(SB, si.nc - j * SB);
+    if (threadIdx.x < SB) xs[threadIdx.x] = (threadIdx.x < bj) ? xload(x + si.a + j * SB + threadIdx.x) : 0.0;
     __syncthreads();
-    if (half == 0 && cc < nb) x[si.a + c0 + cc] = a0 + a1 + part[cc];
+    double tp;
+    if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
+    else tp = slice_dot_stream<QW>(lv, cvo, lso(j), 8u, lvalid(j), live, xs + q * QW);
+    part[q][col] = tp;
+    __syncthreads();
+    if constexpr (PF) {
+      if (j - 1 > blk) slice_load<QW>(l, lv, cvo, lso(j - 1), 8u, lvalid(j - 1), live);
+    }
+    if (q == 0) t -= sum_slices<NQ>(part, col);
+    __syncthreads();
   }
-  signal_add(bflag + sd.pblk_off[s] + blk, bcount + s);
+  if (q == 0) xs[col] = live ? t : 0.0;
+  __syncthreads();
+  double y;
+  if constexpr (PF) y = slice_dot<QW>(w, xs + q * QW);
+  else y = slice_dot_stream<QW>(wv, wvo, wso, 8u, nb - q * QW, live, xs + q * QW);
+  part[q][col] = y;
+  __syncthreads();
+  if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
+  signal_add(sd.bflag + sd.pblk_off[s] + blk, sd.bcount + s);
 }
 
 }  // namespace
 
-void solve_reset(const SolveDev& sd, int ns, hipStream_t st)
+void solve_reset(const SolveDev& sd, hipStream_t st)
 {
-  const int n = 2 * sd.n_pblk + 2 * ns + 1;
+  // every flag but the error word, which stays set until the host reads it
+  const int n = sd.n_flags - 1;
   k_solve_reset<<<(n + 255) / 256, 256, 0, st>>>(sd.flags, n);
   HIPCHK(hipGetLastError());
 }
 
-void solve_forward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st)
+void solve_forward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st)
 {
-  if (sd.n_fwd <= 0) return;
-  k_solve_fwd<<<sd.n_fwd, 256, 0, st>>>(c, sd, x);
+  if (part == 0 && sd.n_fwd > 0)
+    k_solve_fwd<NT_FOREST, false><<<sd.n_fwd, NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, x);
+  if (part == 1 && sd.n_fwd_top > 0)
+    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, x);
   HIPCHK(hipGetLastError());
 }
 
-void solve_backward(const CholDev& c, const SolveDev& sd, double* x, hipStream_t st)
+void solve_backward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st)
 {
-  if (sd.n_bwd <= 0) return;
-  k_solve_bwd<<<sd.n_bwd, 256, 0, st>>>(c, sd, x);
+  if (part == 1 && sd.n_bwd_top > 0)
+    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, 0, st>>>(c, sd, sd.bwd_items_top, x);
+  if (part == 0 && sd.n_bwd > 0)
+    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, x);
   HIPCHK(hipGetLastError());
 }
 

@@ -13,8 +13,11 @@ Profiler::~Profiler()
 
 const char* Profiler::name(int k)
 {
-  static const char* const names[PK_COUNT] = {"k_dense_nt", "k_potrf_inv", "k_extend_add",
-                                              "k_solve_fwd", "k_solve_bwd"};
+  // kernel names as rocprofv3 prints them (template instances of the sweeps)
+  static const char* const names[PK_COUNT] = {
+      "k_dense_nt",           "k_potrf_inv",         "k_extend_add",
+      "k_solve_fwd<256, false>", "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
+      "k_solve_bwd<256, false>"};
   return (k >= 0 && k < PK_COUNT) ? names[k] : "";
 }
 

@@ -12,7 +12,16 @@
 
 namespace hykkt {
 
-enum ProfKind { PK_DENSE_NT = 0, PK_POTRF, PK_EXTEND_ADD, PK_SOLVE_FWD, PK_SOLVE_BWD, PK_COUNT };
+enum ProfKind {
+  PK_DENSE_NT = 0,
+  PK_POTRF,
+  PK_EXTEND_ADD,
+  PK_SOLVE_FWD_FOREST,
+  PK_SOLVE_FWD_TOP,
+  PK_SOLVE_BWD_TOP,
+  PK_SOLVE_BWD_FOREST,
+  PK_COUNT
+};
 
 class Profiler {
  public:

@@ -27,14 +27,17 @@ void apply_s(SchurOps& o, const double* v, double alpha, double beta, const doub
 }  // namespace
 
 int schur_cg_run(SchurOps& o, const double* b, double tol, int itmax, int check_every,
-                 int* iters, double* final_norm)
+                 int* iters, double* final_norm, bool x0_zero)
 {
   const int m = o.m;
   double* sc = o.sc;
-  // setup(): p = s = b (:89-100); r = b - S x0 (:104-122)
+  // setup(): p = s = b (:89-100); r = b - S x0 (:104-122).  With x0 = 0 the
+  // product is exactly zero and fma(-1, 0, b) = b, so r = b bit for bit and
+  // the solve is skipped.
   dev::copy(o.p, b, m, o.st);
   dev::copy(o.s, b, m, o.st);
-  apply_s(o, o.x, -1.0, 1.0, b, o.r);
+  if (x0_zero) dev::copy(o.r, b, m, o.st);
+  else apply_s(o, o.x, -1.0, 1.0, b, o.r);
   dev::dot(m, o.r, o.r, o.part, sc + 0, o.st);
   double h = 0.0;
   HIPCHK(hipMemcpyAsync(&h, sc + 0, sizeof(double), hipMemcpyDeviceToHost, o.st));

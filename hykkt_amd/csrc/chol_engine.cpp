@@ -221,8 +221,7 @@ void CholEngine::build_solve()
   for (int s = 0; s < ns; s++)
     if (top[s])
       for (int p = pl_.sn_parent[s]; p >= 0 && !top[p]; p = pl_.sn_parent[p]) top[p] = 1;
-  std::vector<int> fwd, bwd, fwd_top, bwd_top, pblk(ns + 1, 0);
-  for (int s = 0; s < ns; s++) pblk[s + 1] = pblk[s] + cdiv(pl_.nc(s), NB);
+  std::vector<int> fwd, bwd, fwd_top, bwd_top;
   solve_lower_[0] = solve_lower_[1] = 0.0;
   solve_n_[0] = solve_n_[1] = 0;
   for (int s = 0; s < ns; s++) {   // postorder: children first
@@ -236,7 +235,9 @@ void CholEngine::build_solve()
   std::vector<int64_t> bpo(ns + 1, 0);
   for (int s = ns - 1; s >= 0; s--) {   // parents first: row chunks, then pivot blocks
     std::vector<int>& b = top[s] ? bwd_top : bwd;
-    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++) b.insert(b.end(), {s, 0, u});
+    const int np = cdiv(pl_.nc(s), NB);
+    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++)
+      for (int k = 0; k < np; k++) b.insert(b.end(), {s, 0, u * np + k});
     for (int k = cdiv(pl_.nc(s), NB) - 1; k >= 0; k--) b.insert(b.end(), {s, 1, k});
   }
   for (int s = 0; s < ns; s++)
@@ -245,10 +246,8 @@ void CholEngine::build_solve()
   std::vector<int64_t> foff(ns + 1, 0);
   for (int s = 0; s < ns; s++) foff[s + 1] = foff[s] + pl_.nc(s) + pl_.nr(s);
   std::vector<int> cnt(static_cast<size_t>(foff[ns]) + 1, 0);
-  int64_t uo = 0;
   std::vector<int64_t> uvo(ns + 1, 0);
   for (int s = 0; s < ns; s++) uvo[s + 1] = uvo[s] + pl_.nr(s);
-  (void)uo;
   for (int c = 0; c < ns; c++) {
     int p = pl_.sn_parent[c];
     if (p < 0) continue;
@@ -275,20 +274,17 @@ void CholEngine::build_solve()
   upi(bwd_items_, bwd);
   upi(fwd_top_items_, fwd_top);
   upi(bwd_top_items_, bwd_top);
-  upi(pblk_off_, pblk);
   upi(pull_ptr_, cnt);
   upi(pull_src_, src);
   front_off_.upload(foff);
   bpart_off_.upload(bpo);
   bpart_.alloc(static_cast<size_t>(std::max<int64_t>(bpo[ns], 1)));
-  const int npb = pblk[ns];
-  const int nflags = 2 * npb + 3 * ns + 1;
-  flags_.alloc(nflags);
+  ybuf_.alloc(static_cast<size_t>(std::max(pl_.n, 1)));
+  flags_.alloc(1);
   sd_.fwd_items = fwd_items_.p;
   sd_.bwd_items = bwd_items_.p;
   sd_.fwd_items_top = fwd_top_items_.p;
   sd_.bwd_items_top = bwd_top_items_.p;
-  sd_.pblk_off = pblk_off_.p;
   sd_.uvec_off = uvec_off_.p;
   sd_.pull_ptr = pull_ptr_.p;
   sd_.pull_src = pull_src_.p;
@@ -296,16 +292,9 @@ void CholEngine::build_solve()
   sd_.ubuf = ubuf_.p;
   sd_.bpart = bpart_.p;
   sd_.bpart_off = bpart_off_.p;
-  sd_.flags = flags_.p;
-  sd_.n_flags = nflags;
-  sd_.fflag = flags_.p;
-  sd_.bflag = flags_.p + npb;
-  sd_.fcount = flags_.p + 2 * npb;
-  sd_.bcount = flags_.p + 2 * npb + ns;
-  sd_.ccount = flags_.p + 2 * npb + 2 * ns;
-  sd_.err = flags_.p + 2 * npb + 3 * ns;
-  sd_.n_pblk = npb;
-  dev::fill_i(flags_.p, 0, nflags, st_);
+  sd_.err = flags_.p;
+  n_bpart_ = bpo[ns];
+  dev::fill_i(flags_.p, 0, 1, st_);
 }
 
 void CholEngine::zero_L() { dev::fill(L_.p, 0.0, static_cast<int64_t>(L_.n), st_); }
@@ -384,8 +373,8 @@ int CholEngine::solve_error()
 
 void CholEngine::solve(double* x)
 {
-  dev::solve_reset(sd_, st_);
   Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
+  const int64_t nu = static_cast<int64_t>(ubuf_.n);
   // Algorithmic, per part of the tree: each sweep reads that part's factor
   // entries (lower trapezoids of its panels) once, 2 flops per entry, and
   // reads + writes its vector entries.
@@ -393,13 +382,18 @@ void CholEngine::solve(double* x)
     if ((part == 0 ? (fwd ? sd_.n_fwd : sd_.n_bwd) : (fwd ? sd_.n_fwd_top : sd_.n_bwd_top)) <= 0)
       return;
     if (pf) pf->begin(st_);
-    if (fwd) dev::solve_forward(cd_, sd_, x, part, st_);
-    else dev::solve_backward(cd_, sd_, x, part, st_);
+    if (fwd) dev::solve_forward(cd_, sd_, x, ybuf_.p, part, st_);
+    else dev::solve_backward(cd_, sd_, ybuf_.p, x, part, st_);
     if (pf)
       pf->end(kind, st_, 2.0 * solve_lower_[part], 8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]));
   };
+  // forward: b in x -> y in ybuf; backward: y in ybuf -> x
+  dev::fill_sentinel(ybuf_.p, pl_.n, st_);
+  dev::fill_sentinel(ubuf_.p, nu, st_);
   sweep(PK_SOLVE_FWD_FOREST, 0, true);
   sweep(PK_SOLVE_FWD_TOP, 1, true);
+  dev::fill_sentinel(x, pl_.n, st_);
+  dev::fill_sentinel(bpart_.p, n_bpart_, st_);
   sweep(PK_SOLVE_BWD_TOP, 1, false);
   sweep(PK_SOLVE_BWD_FOREST, 0, false);
 }

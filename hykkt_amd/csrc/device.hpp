@@ -75,8 +75,8 @@ class CholEngine {
   // First failing pivot (internal index) or -1; synchronizes the stream.
   int pivot();
   // Nonzero if a triangular-solve sweep hit its wait bound since the last
-  // call (synchronizes the stream, clears the flag; solves in between leave
-  // it set).
+  // call (synchronizes the stream, clears the error word; solves in between
+  // leave it set).
   int solve_error();
   // Solve L L^T x = b in place, x in internal (postordered) order.
   void solve(double* x);
@@ -116,10 +116,12 @@ class CholEngine {
   DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tr_, t_trsm_tc_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
   DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
   // solve items
-  DevBuf<int> fwd_items_, bwd_items_, pblk_off_, pull_ptr_, pull_src_, flags_;
+  DevBuf<int> fwd_items_, bwd_items_, pull_ptr_, pull_src_, flags_;
   DevBuf<int64_t> bpart_off_;
   DevBuf<double> bpart_;
   DevBuf<int> fwd_top_items_, bwd_top_items_;
+  DevBuf<double> ybuf_;                   // forward-sweep output
+  int64_t n_bpart_ = 0;
   double solve_lower_[2] = {0.0, 0.0};   // stored lower-trapezoid entries: forest, top
   int64_t solve_n_[2] = {0, 0};          // columns: forest, top
 };
@@ -143,7 +145,8 @@ struct SchurOps {
 // the device: ops.x holds x0 on entry and the solution on exit.  The host
 // reads ||r||^2 back every `check_every` iterations for the stopping test.
 // Returns 0 if converged (sqrt(r.r) < tol), 1 after itmax iterations.
+// x0_zero: ops.x is known to be all zeros (r = b without a solve).
 int schur_cg_run(SchurOps& ops, const double* b, double tol, int itmax, int check_every,
-                 int* iters, double* final_norm);
+                 int* iters, double* final_norm, bool x0_zero = false);
 
 }  // namespace hykkt

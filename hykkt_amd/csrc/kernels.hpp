@@ -130,10 +130,10 @@ void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, h
 void dense_nt(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
               const int* task_tc, int mode, int step, hipStream_t st);
 
-// Single-launch supernodal solves over the whole tree (L y = b, then L^T x = y,
-// in place on x, internal order).  Work items are 128-row blocks of the
-// fronts, dispatched in a dependency-respecting order; each item waits for the
-// items it depends on through flags (reset by solve_reset).
+// Single-launch supernodal solves over the whole tree.  Work items are
+// 128-row blocks of the fronts, dispatched in a dependency-respecting order;
+// items synchronise on the values they exchange, whose buffers are filled
+// with a sentinel (fill_sentinel) before each sweep.
 struct SolveDev {
   // Items of the small-supernode forest (fwd_items, bwd_items) and of the top
   // of the tree (*_top) are separate launches: forward forest then top,
@@ -141,33 +141,30 @@ struct SolveDev {
   const int* fwd_items;     // 3 ints per item: supernode, kind (0 pivot, 1 update), block
   int n_fwd;
   const int* bwd_items;     // 3 ints per item: supernode, kind (0 row chunk, 1 pivot), index
+                            // (row chunk: chunk * blocks + column block; pivot: block)
   int n_bwd;
   const int* fwd_items_top;
   int n_fwd_top;
   const int* bwd_items_top;
   int n_bwd_top;
-  const int* pblk_off;      // ns+1: global index of the first pivot block of s
-  const int64_t* uvec_off;  // ns+1 offsets of the forward update vectors
+  const int64_t* uvec_off;  // ns+1 offsets of the forward update vectors in ubuf
   const int* pull_ptr;      // front-row pull lists (children contributions)
   const int* pull_src;
   const int64_t* front_off; // ns+1: offset of s's front rows in the pull_ptr index
-  double* ubuf;
+  double* ubuf;             // forward update vectors (sentinel-filled per sweep)
   double* bpart;            // backward chunk partials: chunk u of s at bpart_off[s] + u*nc
   const int64_t* bpart_off;
-  int* flags;               // all flags below, one allocation, err last
-  int n_flags;
-  int* fflag;               // n_pblk: forward pivot block done
-  int* bflag;               // n_pblk: backward pivot block done
-  int* fcount;              // ns: forward update items done
-  int* bcount;              // ns: backward pivot items done
-  int* ccount;              // ns: backward chunk items done
-  int* err;                 // 1: a wait hit its bound
-  int n_pblk;
+  int* err;                 // 1: a wait hit its bound (sticky until the host clears it)
 };
-void solve_reset(const SolveDev& sd, hipStream_t st);
-// part 0: the forest items, part 1: the top items
-void solve_forward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st);
-void solve_backward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st);
+// p[i] = the sentinel, i < n
+void fill_sentinel(double* p, int64_t n, hipStream_t st);
+// L y = b (b, y internal order; y sentinel-filled, ubuf sentinel-filled).
+// part 0: the forest items, part 1: the top items.
+void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,
+                   hipStream_t st);
+// L^T x = y (x and bpart sentinel-filled).
+void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
+                    hipStream_t st);
 
 }  // namespace dev
 }  // namespace hykkt

@@ -37,8 +37,12 @@ struct GemmArgs {
   double alpha, beta;
 };
 
-// One 128x128 tile of C = beta*C + alpha*A*B^T.  256 threads: 4 waves in a
-// 2x2 arrangement, each owning a 64x64 quadrant = 4x4 MFMA blocks.
+// One 128x128 tile of C = beta*C + alpha*A*B^T with beta in {0, 1} and
+// alpha in {1, -1}.  256 threads: 4 waves in a 2x2 arrangement, each owning a
+// 64x64 quadrant = 4x4 MFMA blocks.  The accumulators start from beta*C (all
+// loads in flight at once) and are stored once at the end; alpha is applied
+// exactly to the staged A values.  K is staged through LDS in chunks of KC,
+// the next chunk's global loads overlapping the current chunk's MFMAs.
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 {
   double* As = smem;                       // [KC][LDA_S]
@@ -47,29 +51,48 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
   const int lane = t & 63, wave = t >> 6;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
   d4 acc[4][4];
+  // acc[i][j][v]: C row = wr + 16 i + lane%16, C col = wc + 16 j + 4 v + lane/16
+  if (g.beta != 0.0) {
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++) {
+      const int r = wr + i * 16 + (lane & 15);
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = wc + j * 16 + 4 * v + (lane >> 4);
+          acc[i][j][v] = (r < g.M && c < g.N) ? g.C[c * g.ldc + r] : 0.0;
+        }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  }
 
   // staging assignment: thread t loads k = t/16, rows (t%16)*8 .. +8
   const int sk = t >> 4, sr = (t & 15) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += KC) {
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
     const int kk = k0 + sk;
-    double ra[8], rb[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      int r = sr + q;
+      const int r = sr + q;
       ra[q] = (kk < g.K && r < g.M) ? g.A[kk * g.lda + r] : 0.0;
       rb[q] = (kk < g.K && r < g.N) ? g.B[kk * g.ldb + r] : 0.0;
     }
-    __syncthreads();
+  };
+  gload(0);
+  for (int k0 = 0; k0 < g.K; k0 += KC) {
+    __syncthreads();   // the previous chunk's fragments have been read
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      As[sk * LDA_S + sr + q] = ra[q];
+      As[sk * LDA_S + sr + q] = g.alpha * ra[q];   // exact: alpha is +-1
       Bs[sk * LDA_S + sr + q] = rb[q];
     }
     __syncthreads();
+    if (k0 + KC < g.K) gload(k0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ks++) {
       const int kr = ks * 4 + (lane >> 4);
@@ -85,21 +108,15 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
     }
   }
-  // acc[i][j][v]: C row = wr + 16 i + lane%16, C col = wc + 16 j + 4 v + lane/16
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = wr + i * 16 + (lane & 15);
-    if (r >= g.M) continue;
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int c = wc + j * 16 + 4 * v + (lane >> 4);
-        if (c < g.N) {
-          double* p = g.C + c * g.ldc + r;
-          double val = g.alpha * acc[i][j][v];
-          *p = (g.beta == 0.0) ? val : g.beta * *p + val;
-        }
+        if (r < g.M && c < g.N) g.C[c * g.ldc + r] = acc[i][j][v];
       }
   }
 }
@@ -195,6 +212,15 @@ __global__ void __launch_bounds__(256) k_dense_nt(CholDev c, const int* __restri
 // follow from X(i,j) = -X(i,i) * sum_{k=j}^{i-1} L(i,k) X(k,j).
 constexpr int SUB = 32;
 
+// Lane `lane`'s value of v, as a wave-uniform (scalar register) value.
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(u), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(u >> 32), lane);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // BK: padded block edge (32, 64 or 128) chosen per launch from the largest
 // diagonal block in it, so small supernodes need little LDS.
 template <int BK>
@@ -212,10 +238,23 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   __shared__ int bad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
-  for (int col = tid / 64; col < BK; col += 4)
-    for (int r = (tid & 63) + col; r < BK; r += 64)
-      P[r][col] = (r < b && col < b) ? D[r + static_cast<int64_t>(col) * si.f]
-                                     : (r == col ? 1.0 : 0.0);
+  // load the block up to 32 elements per thread at a time (all in flight
+  // before the LDS stores): element e = col*BK + r, lower part only
+  constexpr int NE = BK * BK, PER = NE / 256 < 32 ? NE / 256 : 32, STEP = 256 * PER;
+  for (int e0 = tid; e0 < NE; e0 += STEP) {
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = e0 + u * 256, r = e % BK, col = e / BK;
+      v[u] = (e < NE && r >= col && r < b) ? D[r + static_cast<int64_t>(col) * si.f]
+                                           : (r == col ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = e0 + u * 256, r = e % BK, col = e / BK;
+      if (e < NE && r >= col) P[r][col] = v[u];
+    }
+  }
   if (tid == 0) bad = INT_MAX;
   __syncthreads();
 
@@ -223,7 +262,9 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   for (int kb = 0; kb < BK; kb += SUB) {
     if (wave == 0) {
       // lane r holds row r; entries above the diagonal are updated too (they
-      // are never stored), which keeps the loop free of lane-dependent branches
+      // are never stored), which keeps the loop free of lane-dependent branches.
+      // Column k of L reaches every lane through v_readlane (scalar registers),
+      // not through the LDS crossbar.
       const int r = lane & (SUB - 1);
       double v[SUB];
 #pragma unroll
@@ -231,14 +272,14 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
       int badk = -1;
 #pragma unroll
       for (int k = 0; k < SUB; k++) {
-        double dkk = __shfl(v[k], k, 64);
+        const double dkk = readlane_f64(v[k], k);
         const bool ok = (dkk > 0.0) && (sqrt(dkk) > c.pivot_tol);
         badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
         const double d = sqrt(ok ? dkk : 1.0);   // failure reported, arithmetic kept finite
         v[k] = (r == k) ? d : v[k] / d;
         const double vk = v[k];
 #pragma unroll
-        for (int q = k + 1; q < SUB; q++) v[q] -= vk * __shfl(vk, q, 64);
+        for (int q = k + 1; q < SUB; q++) v[q] -= vk * readlane_f64(vk, q);
       }
       if (lane == 0 && badk >= 0 && bad == INT_MAX) bad = si.a + k0 + kb + badk;
       if (lane < SUB) {

@@ -14,9 +14,10 @@
 //   update item (s, u): the same products for 128 rows below s's columns,
 //     written to s's update vector, which the parent pulls.
 // Backward sweep, per supernode from the root down:
-//   chunk item (s, u): once s's parent is done, all of x(rows of s) is final;
-//     the item computes L(rows of chunk u, :)^T x(rows) for every column of s
-//     into a partial-sum buffer (chunks run in parallel);
+//   chunk item (s, u, b): the rows of chunk u of s are ancestors' columns, so
+//     their x is final once computed there; the item computes
+//     L(rows of chunk u, column block b)^T x(rows) into a partial-sum buffer
+//     (all chunks of all column blocks run in parallel);
 //   pivot item (s, b): sums the chunk partials in a fixed order, then
 //     L(j, b)^T x_j over the later pivot blocks j > b as they finish, and
 //     x_b = Dinv_b^T (t - acc).
@@ -28,21 +29,19 @@
 // with a smaller index.  Each XCD dispatches its share of the workgroups in
 // increasing index order, so the smallest unfinished item is always resident
 // and runnable: the sweep cannot deadlock.  Every wait is additionally
-// bounded (SPIN_LIMIT); on expiry the item sets an error flag the host
+// bounded (SPIN_LIMIT); on expiry the item sets an error word the host
 // reports, and carries on, so the kernel always terminates.
 //
-// Cross-workgroup visibility.  The eight XCDs of the chip have separate L2
-// caches, so a device-scope release/acquire fence writes back / invalidates
-// a whole L2, which would cost every link of the chain microseconds and evict
-// the factor blocks other items are streaming.  Instead, every value passed
-// between items (x blocks, update vectors, chunk partials) is written and read
-// with device-scope relaxed atomic stores and loads, which go to the
-// device-coherent level, and the producer waits for its stores to complete
-// (s_waitcnt: on gfx9-family GPUs the vector-memory counter covers stores)
-// before the workgroup barrier that precedes the flag increment.  A consumer
-// observes the flag with a device-scope atomic load, passes a barrier, and
-// only then issues its device-scope loads of the data.  Factor values are
-// plain loads: they were written by earlier kernels.
+// Synchronisation is by the data itself.  Every value passed between items
+// (the forward y blocks, update vectors, backward chunk partials, backward x
+// blocks) lives in a buffer that is filled with a sentinel NaN bit pattern
+// before the sweep -- a pattern arithmetic never produces -- and is written
+// once, with a device-scope atomic store.  A consumer spins on each value it
+// needs with device-scope atomic loads until the sentinel is gone.  Each
+// value is thus published and observed on its own location, with no flags,
+// no fences and no cross-location ordering (the eight XCDs' L2 caches are
+// not coherent with each other; device-scope atomics go to the coherent
+// level).  Factor values are plain loads: earlier kernels wrote them.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -78,35 +77,41 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
   return i;
 }
 
-// Block until *p >= target: thread 0 spins on a device-scope atomic load,
-// then the workgroup passes a barrier.  Gives up (and sets *err) after
-// SPIN_LIMIT polls, or at once if another item already gave up.
-__device__ void wait_geq(int* p, int target, int* err)
+// Sentinel: a signalling-NaN payload no arithmetic result carries.
+constexpr uint64_t SENTINEL = 0x7ff4dead0badf00dull;
+
+__device__ __forceinline__ double xload(const double* p)
 {
-  if (threadIdx.x == 0) {
-    int it = 0;
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++it > SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-  }
-  __syncthreads();
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Publish: every thread waits until its (device-scope atomic) stores have
-// completed, the workgroup passes a barrier, then one thread increments the
-// flag(s).
-__device__ void signal_add(int* p, int* p2 = nullptr)
+__device__ __forceinline__ void xstore(double* p, double v)
 {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (p2) __hip_atomic_fetch_add(p2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool pending(double v)
+{
+  return __builtin_bit_cast(uint64_t, v) == SENTINEL;
+}
+
+// The value at p once its producer has written it.  Gives up (sets *err,
+// returns 0) after SPIN_LIMIT polls, or soon after another wait gave up.
+__device__ double spin_load(const double* p, int* err)
+{
+  double v = xload(p);
+  int it = 0;
+  while (pending(v)) {
+    __builtin_amdgcn_s_sleep(2);
+    ++it;
+    if (it > SPIN_LIMIT ||
+        ((it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return 0.0;
+    }
+    v = xload(p);
   }
+  return v;
 }
 
 // Fixed-order sum of the NQ slice partials of one row (column).
@@ -139,21 +144,11 @@ __device__ __forceinline__ double ld(rsrc_t r, uint32_t voff, uint32_t soff)
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-// Values exchanged between items: device-scope relaxed atomics.
-__device__ __forceinline__ double xload(const double* p)
+__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n)
 {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void xstore(double* p, double v)
-{
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void k_solve_reset(int* flags, int n)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = __builtin_bit_cast(double, SENTINEL);
 }
 
 // A slice of QW values, value k at byte offset vo + so0 + k*sstride of the
@@ -208,7 +203,8 @@ __device__ __forceinline__ double slice_dot_stream(rsrc_t rs, uint32_t vo, uint3
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
-                                                  double* __restrict__ x)
+                                                  const double* __restrict__ b,
+                                                  double* __restrict__ y)
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double yv[SB];
@@ -246,17 +242,14 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     if (kind == 0 && jmax == 0) slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
   }
 
-  // children's update vectors must be complete before they are pulled
-  for (int t = c.child_ptr[s]; t < c.child_ptr[s + 1]; t++) {
-    const int ch = c.child_list[t];
-    const int nu_ch = (c.rows_ptr[ch + 1] - c.rows_ptr[ch] + SB - 1) / SB;
-    wait_geq(sd.fcount + ch, nu_ch, sd.err);
-  }
+  // right-hand side plus the children's update-vector entries for this row
+  // (each waited for as it is read)
   double v = 0.0;
   if (q == 0 && live) {
-    v = (kind == 0) ? x[si.a + r] : 0.0;
+    v = (kind == 0) ? b[si.a + r] : 0.0;
     const int64_t fr = sd.front_off[s] + r;
-    for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++) v += xload(sd.ubuf + sd.pull_src[p]);
+    for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
+      v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
   }
   // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready
   for (int j = 0; j < jmax; j++) {
@@ -264,9 +257,9 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
       if (kind == 0 && j == jmax - 1)   // in flight during the last wait
         slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
     }
-    wait_geq(sd.fflag + sd.pblk_off[s] + j, 1, sd.err);
     const int bj = min(SB, si.nc - j * SB);
-    if (threadIdx.x < SB) yv[threadIdx.x] = (threadIdx.x < bj) ? xload(x + si.a + j * SB + threadIdx.x) : 0.0;
+    if (threadIdx.x < SB)
+      yv[threadIdx.x] = (threadIdx.x < bj) ? spin_load(y + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
     __syncthreads();
     double tp;
     if constexpr (PF) tp = slice_dot<QW>(l, yv + q * QW);
@@ -284,16 +277,14 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     // y = Dinv_blk v
     if (q == 0) yv[row] = live ? v : 0.0;
     __syncthreads();
-    double y;
-    if constexpr (PF) y = slice_dot<QW>(w, yv + q * QW);
-    else y = slice_dot_stream<QW>(wv, wvo, wso, SB * 8u, nrows - q * QW, live, yv + q * QW);
-    part[q][row] = y;
+    double res;
+    if constexpr (PF) res = slice_dot<QW>(w, yv + q * QW);
+    else res = slice_dot_stream<QW>(wv, wvo, wso, SB * 8u, nrows - q * QW, live, yv + q * QW);
+    part[q][row] = res;
     __syncthreads();
-    if (q == 0 && live) xstore(x + si.a + r, sum_slices<NQ>(part, row));
-    signal_add(sd.fflag + sd.pblk_off[s] + blk);
+    if (q == 0 && live) xstore(y + si.a + r, sum_slices<NQ>(part, row));
   } else {
     if (q == 0 && live) xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v);
-    signal_add(sd.fcount + s);
   }
 }
 
@@ -301,6 +292,7 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
+                                                  const double* __restrict__ y,
                                                   double* __restrict__ x)
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
@@ -320,42 +312,28 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   const uint32_t cvo = static_cast<uint32_t>(col) * f8;
 
   if (kind == 0) {
-    // ---- chunk item: rows [nc + idx*128, ...) of s, every column block
-    const int rr0 = idx * SB;
+    // ---- chunk item (u, b): rows [nc + u*128, ...) of s against column
+    // block b: partial L(rows, b-block)^T x(rows)
+    const int u = idx / si.np, b = idx - u * si.np;
+    const int rr0 = u * SB;
     const int nrows = min(SB, si.nr - rr0);
+    const int bb = min(SB, si.nc - b * SB);
     const int* rows = c.rows + c.rows_ptr[s] + rr0;
     const uint32_t so = static_cast<uint32_t>(si.nc + rr0 + q * QW) * 8u;
+    const rsrc_t lv = cview(b * SB, bb);
     double l[PF ? QW : 1];
-    if constexpr (PF) {
-      const int nb0 = min(SB, si.nc);
-      slice_load<QW>(l, cview(0, nb0), cvo, so, 8u, nrows - q * QW, col < nb0);
-    }
-    const int p = c.sn_parent[s];
-    if (p >= 0) {
-      const int np_p = (c.sn_first[p + 1] - c.sn_first[p] + SB - 1) / SB;
-      wait_geq(sd.bcount + p, np_p, sd.err);
-    }
-    if (threadIdx.x < SB) xs[threadIdx.x] = (threadIdx.x < nrows) ? xload(x + rows[threadIdx.x]) : 0.0;
+    if constexpr (PF) slice_load<QW>(l, lv, cvo, so, 8u, nrows - q * QW, col < bb);
+    // x of these rows belongs to ancestors: waited for as it is read
+    if (threadIdx.x < SB)
+      xs[threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
     __syncthreads();
-    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(idx) * si.nc;
-    for (int b = 0; b < si.np; b++) {
-      const int bb = min(SB, si.nc - b * SB);
-      double tp;
-      if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
-      else tp = slice_dot_stream<QW>(cview(b * SB, bb), cvo, so, 8u, nrows - q * QW, col < bb,
-                                     xs + q * QW);
-      part[q][col] = tp;
-      __syncthreads();
-      if constexpr (PF) {
-        if (b + 1 < si.np) {
-          const int nbn = min(SB, si.nc - (b + 1) * SB);
-          slice_load<QW>(l, cview((b + 1) * SB, nbn), cvo, so, 8u, nrows - q * QW, col < nbn);
-        }
-      }
-      if (q == 0 && col < bb) xstore(out + b * SB + col, sum_slices<NQ>(part, col));
-      __syncthreads();
-    }
-    signal_add(sd.ccount + s);
+    double tp;
+    if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
+    else tp = slice_dot_stream<QW>(lv, cvo, so, 8u, nrows - q * QW, col < bb, xs + q * QW);
+    part[q][col] = tp;
+    __syncthreads();
+    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
+    if (q == 0 && col < bb) xstore(out + col, sum_slices<NQ>(part, col));
     return;
   }
 
@@ -379,20 +357,19 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     else slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);
   }
 
-  if (si.nu > 0) wait_geq(sd.ccount + s, si.nu, sd.err);
   double t = 0.0;
   if (q == 0 && live) {
-    t = x[si.a + c0 + col];
+    t = y[si.a + c0 + col];
     const double* bp = sd.bpart + sd.bpart_off[s] + c0 + col;
-    for (int u = 0; u < si.nu; u++) t -= xload(bp + static_cast<int64_t>(u) * si.nc);
+    for (int u = 0; u < si.nu; u++) t -= spin_load(bp + static_cast<int64_t>(u) * si.nc, sd.err);
   }
   for (int j = jtop; j > blk; j--) {
     if constexpr (PF) {
       if (j == blk + 1) slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);   // last wait
     }
-    wait_geq(sd.bflag + sd.pblk_off[s] + j, 1, sd.err);
     const int bj = min(SB, si.nc - j * SB);
-    if (threadIdx.x < SB) xs[threadIdx.x] = (threadIdx.x < bj) ? xload(x + si.a + j * SB + threadIdx.x) : 0.0;
+    if (threadIdx.x < SB)
+      xs[threadIdx.x] = (threadIdx.x < bj) ? spin_load(x + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
     __syncthreads();
     double tp;
     if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
@@ -407,40 +384,42 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   }
   if (q == 0) xs[col] = live ? t : 0.0;
   __syncthreads();
-  double y;
-  if constexpr (PF) y = slice_dot<QW>(w, xs + q * QW);
-  else y = slice_dot_stream<QW>(wv, wvo, wso, 8u, nb - q * QW, live, xs + q * QW);
-  part[q][col] = y;
+  double res;
+  if constexpr (PF) res = slice_dot<QW>(w, xs + q * QW);
+  else res = slice_dot_stream<QW>(wv, wvo, wso, 8u, nb - q * QW, live, xs + q * QW);
+  part[q][col] = res;
   __syncthreads();
   if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
-  signal_add(sd.bflag + sd.pblk_off[s] + blk, sd.bcount + s);
 }
 
 }  // namespace
 
-void solve_reset(const SolveDev& sd, hipStream_t st)
+void fill_sentinel(double* p, int64_t n, hipStream_t st)
 {
-  // every flag but the error word, which stays set until the host reads it
-  const int n = sd.n_flags - 1;
-  k_solve_reset<<<(n + 255) / 256, 256, 0, st>>>(sd.flags, n);
+  if (n <= 0) return;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p, n);
   HIPCHK(hipGetLastError());
 }
 
-void solve_forward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st)
+void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,
+                   hipStream_t st)
 {
   if (part == 0 && sd.n_fwd > 0)
-    k_solve_fwd<NT_FOREST, false><<<sd.n_fwd, NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, x);
+    k_solve_fwd<NT_FOREST, false><<<sd.n_fwd, NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, b, y);
   if (part == 1 && sd.n_fwd_top > 0)
-    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, x);
+    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, b, y);
   HIPCHK(hipGetLastError());
 }
 
-void solve_backward(const CholDev& c, const SolveDev& sd, double* x, int part, hipStream_t st)
+void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
+                    hipStream_t st)
 {
   if (part == 1 && sd.n_bwd_top > 0)
-    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, 0, st>>>(c, sd, sd.bwd_items_top, x);
+    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, 0, st>>>(c, sd, sd.bwd_items_top, y, x);
   if (part == 0 && sd.n_bwd > 0)
-    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, x);
+    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, y, x);
   HIPCHK(hipGetLastError());
 }
 

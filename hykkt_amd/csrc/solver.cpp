@@ -460,8 +460,9 @@ int Solver::execute()
   Pass first{S.rxp.p, S.schur.p, S.cx.p};
   int iters = 0, cg_status = 0;
   double fnorm = 0.0;
-  solve_phase(first, S.rx_til.p, S.ry_s.p, S.drs, S.dryd, S.y0.p, S.dx.p, S.dsv.p, S.dyc.p,
-              S.dyd.p, &iters, &fnorm, &cg_status, ev_[2]);
+  // CG start y_host, or zero when none is bound (then no start-residual solve)
+  solve_phase(first, S.rx_til.p, S.ry_s.p, S.drs, S.dryd, vecs_[VEC_Y] ? S.y0.p : nullptr, S.dx.p,
+              S.dsv.p, S.dyc.p, S.dyd.p, &iters, &fnorm, &cg_status, ev_[2]);
 
   // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
   double hr[8];
@@ -580,8 +581,8 @@ void Solver::solve_phase(const Pass& w, const double* rxt, const double* ysc, co
   ops.part = S.part.p;
   if (x0) dev::copy(w.cx, x0, m, st_);
   else dev::fill(w.cx, 0.0, m, st_);
-  *cg_status =
-      schur_cg_run(ops, w.schur, opts.cg_tol, opts.cg_itmax, opts.cg_check_every, iters, fnorm);
+  *cg_status = schur_cg_run(ops, w.schur, opts.cg_tol, opts.cg_itmax, opts.cg_check_every, iters,
+                            fnorm, x0 == nullptr);
   if (after_cg) HIPCHK(hipEventRecord(after_cg, st_));
 
   dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, -1.0, w.cx, 1.0, w.rxp, S.t1.p,

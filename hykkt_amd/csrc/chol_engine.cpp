@@ -23,6 +23,9 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st) : st_(st)
 {
+  HIPCHK(hipStreamCreate(&st2_));   // default priority, like the solver's stream
+  HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming));
   pl_ = chol_analyze(a, opt);
   auto upi = [](DevBuf<int>& b, const std::vector<int>& v) {
     if (v.empty()) b.upload(std::vector<int>{0});
@@ -79,6 +82,13 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   build_solve();
 }
 
+CholEngine::~CholEngine()
+{
+  if (ev_main_) (void)hipEventDestroy(ev_main_);
+  if (ev_rest_) (void)hipEventDestroy(ev_rest_);
+  if (st2_) (void)hipStreamDestroy(st2_);
+}
+
 void CholEngine::build_tasks()
 {
   const int nlev = static_cast<int>(pl_.level_ptr.size()) - 1;
@@ -130,11 +140,11 @@ void CholEngine::build_tasks()
     for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, cdiv(pl_.nc(pl_.level_sn[q]), NB));
     L.potrf.resize(nsteps);
     L.trsm.resize(nsteps);
-    L.upd.resize(nsteps);
+    L.upd_next.resize(nsteps);
+    L.upd_rest.resize(nsteps);
     for (int j = 0; j < nsteps; j++) {
       L.potrf[j].off = static_cast<int>(potrf.size());
       L.trsm[j].off = static_cast<int>(trsm_sn.size());
-      L.upd[j].off = static_cast<int>(upd_sn.size());
       for (int q = b0; q < b1; q++) {
         const int s = pl_.level_sn[q];
         const int nc = pl_.nc(s), f = nc + pl_.nr(s);
@@ -154,23 +164,41 @@ void CholEngine::build_tasks()
         L.potrf[j].bytes += 16.0 * tri;
         L.trsm[j].flops += r * db * db;
         L.trsm[j].bytes += 8.0 * (2 * r * db + tri);
-        L.upd[j].flops += 2.0 * db * pairs;
-        L.upd[j].bytes += 8.0 * (r * db + 2 * pairs);
+        // the update splits into the next block column (tile column 0) and
+        // the rest, so the next step can start before the rest is done
+        const double cn = std::min<double>(NB, cc);
+        const double pn = cn * r - cn * (cn - 1) / 2;
+        L.upd_next[j].flops += 2.0 * db * pn;
+        L.upd_next[j].bytes += 8.0 * (r * db + 2 * pn);
+        L.upd_rest[j].flops += 2.0 * db * (pairs - pn);
+        L.upd_rest[j].bytes += 8.0 * 2 * (pairs - pn);
+        (void)ctiles;
         for (int t = 0; t < rtiles; t++) {
           trsm_sn.push_back(s);
           trsm_tr.push_back(t);
           trsm_tc.push_back(0);
         }
-        for (int tc = 0; tc < ctiles; tc++)
-          for (int tr = tc; tr < rtiles; tr++) {
-            upd_sn.push_back(s);
-            upd_tr.push_back(tr);
-            upd_tc.push_back(tc);
-          }
       }
       L.potrf[j].cnt = static_cast<int>(potrf.size()) - L.potrf[j].off;
       L.trsm[j].cnt = static_cast<int>(trsm_sn.size()) - L.trsm[j].off;
-      L.upd[j].cnt = static_cast<int>(upd_sn.size()) - L.upd[j].off;
+      for (int part = 0; part < 2; part++) {
+        Launch& la = part == 0 ? L.upd_next[j] : L.upd_rest[j];
+        la.off = static_cast<int>(upd_sn.size());
+        for (int q = b0; q < b1; q++) {
+          const int s = pl_.level_sn[q];
+          const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
+          if (k0 >= nc) continue;
+          const int b = std::min(NB, nc - k0);
+          const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
+          for (int tc = part == 0 ? 0 : 1; tc < (part == 0 ? std::min(1, ctiles) : ctiles); tc++)
+            for (int tr = tc; tr < rtiles; tr++) {
+              upd_sn.push_back(s);
+              upd_tr.push_back(tr);
+              upd_tc.push_back(tc);
+            }
+        }
+        la.cnt = static_cast<int>(upd_sn.size()) - la.off;
+      }
     }
     L.syrk.off = static_cast<int>(syrk_sn.size());
     for (int q = b0; q < b1; q++) {
@@ -316,18 +344,27 @@ void CholEngine::factor(double pivot_tol)
   dev::fill_i(pivot_.p, INT_MAX, 1, st_);
   Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
   // Bracket one launch (skipped when it has no tasks, like the launchers).
-  auto timed = [&](int kind, const Launch& la, auto&& launch) {
+  auto timed_on = [&](hipStream_t st, int kind, const Launch& la, auto&& launch) {
     if (la.cnt <= 0) return;
-    if (pf) pf->begin(st_);
+    if (pf) pf->begin(st);
     launch();
-    if (pf) pf->end(kind, st_, la.flops, la.bytes);
+    if (pf) pf->end(kind, st, la.flops, la.bytes);
   };
+  auto timed = [&](int kind, const Launch& la, auto&& launch) { timed_on(st_, kind, la, launch); };
   for (const Level& L : lv_) {
     timed(PK_EXTEND_ADD, L.ea_panel, [&] {
       dev::extend_add(cd_, L.ea_panel.cnt, t_ea_p_.p + L.ea_panel.off,
                       t_ea_col_.p + L.ea_panel.off, t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p,
                       t_ea_ccol_.p, st_);
     });
+    // Blocked steps with look-ahead.  Main stream: diagonal blocks, TRSM,
+    // update of the next block column; second stream: the rest of the
+    // trailing update, which overlaps the next step's diagonal block and
+    // TRSM.  The next-column update of step j writes columns the rest of
+    // step j-1 also writes, so it waits for that (ev_rest_); the rest of
+    // step j waits for step j's TRSM (ev_main_).  Both events are re-recorded
+    // every step after their waits were enqueued.
+    bool rest_pending = false;
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
       timed(PK_POTRF, L.potrf[j], [&] {
@@ -338,11 +375,26 @@ void CholEngine::factor(double pivot_tol)
         dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
                       t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
       });
-      timed(PK_DENSE_NT, L.upd[j], [&] {
-        dev::dense_nt(cd_, L.upd[j].cnt, t_upd_sn_.p + L.upd[j].off, t_upd_tr_.p + L.upd[j].off,
-                      t_upd_tc_.p + L.upd[j].off, 0, step, st_);
+      if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
+      rest_pending = false;
+      timed(PK_DENSE_NT, L.upd_next[j], [&] {
+        dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
+                      t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, 0, step,
+                      st_);
       });
+      if (L.upd_rest[j].cnt > 0) {
+        HIPCHK(hipEventRecord(ev_main_, st_));
+        HIPCHK(hipStreamWaitEvent(st2_, ev_main_, 0));
+        timed_on(st2_, PK_DENSE_NT, L.upd_rest[j], [&] {
+          dev::dense_nt(cd_, L.upd_rest[j].cnt, t_upd_sn_.p + L.upd_rest[j].off,
+                        t_upd_tr_.p + L.upd_rest[j].off, t_upd_tc_.p + L.upd_rest[j].off, 0, step,
+                        st2_);
+        });
+        HIPCHK(hipEventRecord(ev_rest_, st2_));
+        rest_pending = true;
+      }
     }
+    if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
     timed(PK_DENSE_NT, L.syrk, [&] {
       dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
                     t_syrk_tc_.p + L.syrk.off, 1, 0, st_);

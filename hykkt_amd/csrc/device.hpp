@@ -62,6 +62,9 @@ class CholEngine {
  public:
   // `a`: full n x n CSR whose lower triangle is factored.
   CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st);
+  ~CholEngine();
+  CholEngine(const CholEngine&) = delete;
+  CholEngine& operator=(const CholEngine&) = delete;
   const CholPlan& plan() const { return pl_; }
   double* L() { return L_.p; }
   // Zero the factor storage (before assembly of the matrix entries).
@@ -95,13 +98,15 @@ class CholEngine {
   };
   struct Level {
     Launch ea_panel, ea_u, syrk;
-    std::vector<Launch> potrf, trsm, upd;
+    std::vector<Launch> potrf, trsm, upd_next, upd_rest;
   };
   void build_tasks();
   void build_solve();
 
   CholPlan pl_;
   hipStream_t st_;
+  hipStream_t st2_ = nullptr;   // look-ahead: the rest of each trailing update
+  hipEvent_t ev_main_ = nullptr, ev_rest_ = nullptr;
   Profiler* prof_ = nullptr;
   dev::CholDev cd_{};
   dev::SolveDev sd_{};

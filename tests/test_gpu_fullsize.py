@@ -90,6 +90,9 @@ def test_kernel_timing_accounting(seq):
         assert kt["k_solve_fwd" + part]["launches"] == kt["k_solve_bwd" + part]["launches"] >= 3
     fl = kt["k_dense_nt"]["flops"] + kt["k_potrf_inv"]["flops"]
     assert 0.999 <= fl / st["factor_flops"] <= 1.2, fl / st["factor_flops"]
-    # event-timed kernels fit inside the solve's own wall span
-    total = sum(v["ms"] for v in kt.values())
-    assert total <= st["t_solve_ms"] * 1.01
+    # event-timed kernels fit inside the solve's own wall span: each kernel's
+    # sum, and the total, which may count the look-ahead stream's updates
+    # twice over the time they overlap the main stream
+    for k, v in kt.items():
+        assert v["ms"] <= st["t_solve_ms"] * 1.01, k
+    assert sum(v["ms"] for v in kt.values()) <= 2.0 * st["t_solve_ms"]

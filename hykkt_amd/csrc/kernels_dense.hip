@@ -199,17 +199,23 @@ __global__ void __launch_bounds__(256) k_dense_nt(CholDev c, const int* __restri
 
 // Factor the diagonal block of step `step` (b <= 128) in LDS, write L_kk back
 // and its inverse (lower triangular, zero-padded to 128 x 128, column-major)
-// to dinv.  LDS holds one 128 x 129 array P: L(r, c) (r >= c) at P[r][c]; the
-// inverse X(i, j) (i > j) at P[j][i] and X(j, j) at P[j][128]; plus a 3 x 32 x 33
-// scratch for the off-diagonal blocks of the inverse.  The block is padded to
-// 128 with the identity.
+// to dinv.  LDS holds one BK x (BK+1) array P: L(r, c) (r >= c) at P[r][c];
+// the inverse X(i, j) (i > j) at P[j][i] and X(j, j) at P[j][BK]; the
+// reciprocals of L's diagonal; and 32 x 33 scratch blocks for the inverse's
+// off-diagonal products (one per block pair of a block diagonal).  The block is padded to BK with the
+// identity.
 //
-// Both steps are blocked by 32.  Cholesky: wave 0 factors the 32x32 diagonal
-// block in registers (lane r holds row r; columns move between lanes by
-// shuffles), then every thread solves one row below against it, then the
-// trailing lower triangle is updated.  Inverse: wave w inverts diagonal block
-// w in registers (lane j computes column j), then the block diagonals below
-// follow from X(i,j) = -X(i,i) * sum_{k=j}^{i-1} L(i,k) X(k,j).
+// Both steps are blocked by 32.  Cholesky, per 32-column panel: wave 0
+// factors the 32x32 diagonal block in registers (lane r holds row r; column
+// k reaches all lanes through v_readlane), every thread solves one row below
+// against it (multiplying by reciprocals, no divisions on the chain), and the
+// trailing lower part is updated with FP64 MFMA 16x16x4 tiles.  Inverse: wave
+// w inverts diagonal block w (lane j computes column j), then the block
+// diagonals below follow from X(i,j) = -X(i,i) * sum_{k=j}^{i-1} L(i,k) X(k,j),
+// both products on the matrix cores.
+//
+// MFMA f64 16x16x4 (lane l of 64): A operand A(l%16, l/16), B operand
+// B(l/16, l%16), result element v = D(4v + l/16, l%16) of D = A B.
 constexpr int SUB = 32;
 
 // Lane `lane`'s value of v, as a wave-uniform (scalar register) value.
@@ -221,12 +227,44 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// One 16x16 tile of D = A B^T-style products accumulated over K (a multiple
+// of 4): a_at(i, k) and b_at(k, j) give the operands.
+template <class FA, class FB>
+__device__ __forceinline__ d4 mfma16(int K, int lane, FA a_at, FB b_at)
+{
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int i = lane & 15, kq = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a_at(i, k0 + kq), b_at(k0 + kq, i), acc, 0, 0, 0);
+  return acc;
+}
+
+// A 32x32 block of D = A B as four 16x16 tiles with independent accumulators
+// (interleaved, so the matrix core pipelines them): acc[2*ti + tj] holds
+// tile (ti, tj), element v at D(16 ti + 4v + lane/16, 16 tj + lane%16).
+template <class FA, class FB>
+__device__ __forceinline__ void mfma32(int K, int lane, FA a_at, FB b_at, d4 (&acc)[4])
+{
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+  const int i = lane & 15, kq = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const double a0 = a_at(i, k0 + kq), a1 = a_at(16 + i, k0 + kq);
+    const double b0 = b_at(k0 + kq, i), b1 = b_at(k0 + kq, 16 + i);
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+  }
+}
+
 // BK: padded block edge (32, 64 or 128) chosen per launch from the largest
 // diagonal block in it, so small supernodes need little LDS.
 template <int BK>
 __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
                                                    int step)
 {
+  constexpr int NB = BK / SUB;   // 32-wide panels
   extern __shared__ double smem[];
   const int s = sns[blockIdx.x];
   const SnInfo si = sninfo(c, s);
@@ -234,12 +272,13 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   if (k0 >= si.nc) return;
   const int b = min(BK, si.nc - k0);
   double (*P)[BK + 1] = reinterpret_cast<double (*)[BK + 1]>(smem);
-  double (*T)[SUB][SUB + 1] = reinterpret_cast<double (*)[SUB][SUB + 1]>(smem + BK * (BK + 1));
+  double* rd = smem + BK * (BK + 1);                                   // 1 / L(r, r)
+  double (*T)[SUB][SUB + 1] = reinterpret_cast<double (*)[SUB][SUB + 1]>(rd + BK);
   __shared__ int bad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
-  // load the block up to 32 elements per thread at a time (all in flight
-  // before the LDS stores): element e = col*BK + r, lower part only
+  // load the block, up to 32 elements per thread in flight: element
+  // e = col*BK + r, lower part only, padded with the identity
   constexpr int NE = BK * BK, PER = NE / 256 < 32 ? NE / 256 : 32, STEP = 256 * PER;
   for (int e0 = tid; e0 < NE; e0 += STEP) {
     double v[PER];
@@ -258,193 +297,145 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   if (tid == 0) bad = INT_MAX;
   __syncthreads();
 
-  // ---- blocked Cholesky ----
-  for (int kb = 0; kb < BK; kb += SUB) {
+  // X(i, j) for i >= j from its storage; 0 above the diagonal
+  auto X = [&](int i, int j) -> double { return (i > j) ? P[j][i] : ((i == j) ? P[j][BK] : 0.0); };
+  const double tol2 = c.pivot_tol * c.pivot_tol;
+
+  // ---- blocked Cholesky; the diagonal blocks of the inverse on the way ----
+#pragma unroll 1
+  for (int p = 0; p < NB; p++) {
+    const int kb = p * SUB;
     if (wave == 0) {
-      // lane r holds row r; entries above the diagonal are updated too (they
-      // are never stored), which keeps the loop free of lane-dependent branches.
-      // Column k of L reaches every lane through v_readlane (scalar registers),
-      // not through the LDS crossbar.
+      // lane r holds row r of the diagonal block; entries above the diagonal
+      // are carried along (never stored), so no lane-dependent branches.
+      // Step k: pivot from lane k, 1/sqrt from v_rsq_f64 refined by two
+      // Newton steps, column k to every lane through v_readlane.
       const int r = lane & (SUB - 1);
       double v[SUB];
 #pragma unroll
-      for (int q = 0; q < SUB; q++) v[q] = (q <= r) ? P[kb + r][kb + q] : 0.0;
+      for (int q = 0; q < SUB; q++) v[q] = P[kb + r][kb + q];
       int badk = -1;
+      double myinv = 1.0;   // 1 / L(r, r), taken at step k == r (no dynamic register index)
 #pragma unroll
       for (int k = 0; k < SUB; k++) {
         const double dkk = readlane_f64(v[k], k);
-        const bool ok = (dkk > 0.0) && (sqrt(dkk) > c.pivot_tol);
+        const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
         badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
-        const double d = sqrt(ok ? dkk : 1.0);   // failure reported, arithmetic kept finite
-        v[k] = (r == k) ? d : v[k] / d;
+        const double xk = ok ? dkk : 1.0;   // failure reported, arithmetic kept finite
+        double y = __builtin_amdgcn_rsq(xk);
+        const double hx = 0.5 * xk;
+        y = y * fma(-hx, y * y, 1.5);
+        y = y * fma(-hx, y * y, 1.5);
+        v[k] = (r == k) ? xk * y : v[k] * y;
+        myinv = (r == k) ? y : myinv;
         const double vk = v[k];
 #pragma unroll
         for (int q = k + 1; q < SUB; q++) v[q] -= vk * readlane_f64(vk, q);
       }
-      if (lane == 0 && badk >= 0 && bad == INT_MAX) bad = si.a + k0 + kb + badk;
+      if (lane == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
       if (lane < SUB) {
 #pragma unroll
         for (int q = 0; q < SUB; q++)
           if (q <= r) P[kb + r][kb + q] = v[q];
+        rd[kb + r] = myinv;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // inverse of this diagonal block: lane j < 32 computes column j;
+      // X(i, j) (i > j) goes to P[j][i], X(j, j) to P[j][BK]
+      if (lane < SUB) {
+        const int j = lane;
+        double x[SUB];
+#pragma unroll
+        for (int i = 0; i < SUB; i++) {
+          double t0 = (i == j) ? 1.0 : 0.0, t1 = 0.0;
+#pragma unroll
+          for (int k = 0; k < i; k++) {
+            if (k & 1) t1 -= P[kb + i][kb + k] * x[k];
+            else t0 -= P[kb + i][kb + k] * x[k];
+          }
+          x[i] = (i >= j) ? (t0 + t1) * rd[kb + i] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < SUB; i++) {
+          if (i == j) P[kb + j][BK] = x[i];
+          else if (i > j) P[kb + j][kb + i] = x[i];
+        }
       }
     }
     __syncthreads();
     const int rbeg = kb + SUB;
     if (rbeg < BK) {
-      // rows below: x = a * L_kb^{-T}, one row per thread, row held in registers
-      for (int r = rbeg + tid; r < BK; r += blockDim.x) {
-        double x[SUB];
+      // rows below: L(r, panel) = A(r, panel) X_pp^T on the matrix cores,
+      // 16-row x 16-column tiles; all reads before the in-place writes
+      const int nrt = (BK - rbeg) / 16, ntl = 2 * nrt;
+      d4 acc[3];
 #pragma unroll
-        for (int q = 0; q < SUB; q++) x[q] = P[r][kb + q];
-#pragma unroll
-        for (int k = 0; k < SUB; k++) {
-          double t0 = x[k], t1 = 0.0;
-#pragma unroll
-          for (int q = 0; q < k; q++) {
-            if (q & 1) t1 -= x[q] * P[kb + k][kb + q];
-            else t0 -= x[q] * P[kb + k][kb + q];
-          }
-          x[k] = (t0 + t1) / P[kb + k][kb + k];
+      for (int u = 0; u < 3; u++) {
+        const int t = wave + 4 * u;
+        if (t < ntl) {
+          const int r0 = rbeg + 16 * (t >> 1), c0 = 16 * (t & 1);
+          acc[u] = mfma16(SUB, lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
+                          [&](int k, int j) { return X(kb + c0 + j, kb + k); });
         }
-#pragma unroll
-        for (int q = 0; q < SUB; q++) P[r][kb + q] = x[q];
       }
       __syncthreads();
-      // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k], r >= cc >= rbeg,
-      // in 4x4 register tiles (lower tiles only)
-      const int mt = (BK - rbeg) / 4;
-      for (int e = tid; e < mt * mt; e += blockDim.x) {
-        const int ti = e / mt, tj = e % mt;
-        if (tj > ti) continue;
-        const int r0 = rbeg + 4 * ti, c0 = rbeg + 4 * tj;
-        double acc[4][4] = {};
-#pragma unroll 4
-        for (int k = kb; k < kb + SUB; k++) {
-          double a[4], bb[4];
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            a[q] = P[r0 + q][k];
-            bb[q] = P[c0 + q][k];
-          }
+      for (int u = 0; u < 3; u++) {
+        const int t = wave + 4 * u;
+        if (t < ntl) {
+          const int r0 = rbeg + 16 * (t >> 1), c0 = 16 * (t & 1);
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int w = 0; w < 4; w++) acc[q][w] += a[q] * bb[w];
+          for (int v = 0; v < 4; v++) P[r0 + 4 * v + (lane >> 4)][kb + c0 + (lane & 15)] = acc[u][v];
         }
+      }
+      __syncthreads();
+      // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k] over the panel,
+      // r >= cc >= rbeg, in 16x16 MFMA tiles (lower tiles only; the upper
+      // half of diagonal tiles is scratch)
+      const int mt = (BK - rbeg) / 16;
+      for (int e = wave; e < mt * (mt + 1) / 2; e += 4) {
+        int ti = 0;
+        while ((ti + 1) * (ti + 2) / 2 <= e) ti++;
+        const int tj = e - ti * (ti + 1) / 2;
+        const int r0 = rbeg + 16 * ti, c0 = rbeg + 16 * tj;
+        const d4 a = mfma16(SUB, lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
+                            [&](int k, int j) { return P[c0 + j][kb + k]; });
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-          for (int w = 0; w < 4; w++)
-            if (c0 + w <= r0 + q) P[r0 + q][c0 + w] -= acc[q][w];
+        for (int v = 0; v < 4; v++) P[r0 + 4 * v + (lane >> 4)][c0 + (lane & 15)] -= a[v];
       }
       __syncthreads();
     }
   }
 
-  // ---- inverse of L ----
-  {
-    // wave w inverts diagonal block w; lane j < 32 computes column j
-    const int base = wave * SUB, j = lane;
-    if (j < SUB && base < BK) {
-      double x[SUB];
+  // block diagonals d = 1 .. NB-1: pair q (rows ib = (q+d)*32, cols jb = q*32) on wave q
+#pragma unroll 1
+  for (int d = 1; d < NB; d++) {
+    const int q = wave;
+    const bool act = q < NB - d;
+    const int ib = (q + d) * SUB, jb = q * SUB;
+    d4 acc[4];
+    if (act) {
+      // T = sum_{k = jb}^{ib-1} L(ib.., k) X(k, jb..)   (32 x 32, K = 32 d)
+      mfma32(SUB * d, lane, [&](int i, int k) { return P[ib + i][jb + k]; },
+             [&](int k, int j) { return X(jb + k, jb + j); }, acc);
 #pragma unroll
-      for (int i = 0; i < SUB; i++) {
-        double t0 = (i == j) ? 1.0 : 0.0, t1 = 0.0;
+      for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int k = 0; k < i; k++) {
-          if (k & 1) t1 -= P[base + i][base + k] * x[k];
-          else t0 -= P[base + i][base + k] * x[k];
-        }
-        x[i] = (i >= j) ? (t0 + t1) / P[base + i][base + i] : 0.0;
-      }
-#pragma unroll
-      for (int i = 0; i < SUB; i++) {
-        if (i == j) P[base + j][BK] = x[i];
-        else if (i > j) P[base + j][base + i] = x[i];
-      }
-    }
-  }
-  __syncthreads();
-  auto X = [&](int i, int j) -> double { return (i == j) ? P[j][BK] : P[j][i]; };
-  // block diagonals: T_q = sum_{k=j..ib-1} L(i,k) X(k,j), then X(i,j) = -X_ii T_q,
-  // 4x4 register tiles, (q, tile) spread over the threads
-  for (int d = 1; d < BK / SUB; d++) {
-    const int nblk = BK / SUB - d;
-    constexpr int TPB4 = (SUB / 4) * (SUB / 4);   // 4x4 tiles per 32x32 block
-    for (int e = tid; e < nblk * TPB4; e += blockDim.x) {
-      const int q = e / TPB4, ti = (e % TPB4) / (SUB / 4), tj = e % (SUB / 4);
-      const int ib = (q + d) * SUB, i0 = ib + 4 * ti, j0 = q * SUB + 4 * tj;
-      double acc[4][4] = {};
-      // X(k, j) = 0 for k < j: start at the tile's first column; the first
-      // four k straddle the diagonal of X, beyond them X(k, j) = P[j][k]
-#pragma unroll
-      for (int k = j0; k < j0 + 4; k++) {
-        double a[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          a[u] = P[i0 + u][k];
-          const int j = j0 + u;
-          xv[u] = (k > j) ? P[j][k] : ((k == j) ? P[j][BK] : 0.0);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int w = 0; w < 4; w++) acc[u][w] += a[u] * xv[w];
-      }
-      for (int k = j0 + 4; k < ib; k++) {
-        double a[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          a[u] = P[i0 + u][k];
-          xv[u] = P[j0 + u][k];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int w = 0; w < 4; w++) acc[u][w] += a[u] * xv[w];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int w = 0; w < 4; w++) T[q][4 * ti + u][4 * tj + w] = acc[u][w];
+        for (int v = 0; v < 4; v++)
+          T[q][16 * (t >> 1) + 4 * v + (lane >> 4)][16 * (t & 1) + (lane & 15)] = acc[t][v];
     }
     __syncthreads();
-    for (int e = tid; e < nblk * TPB4; e += blockDim.x) {
-      const int q = e / TPB4, ti = (e % TPB4) / (SUB / 4), tj = e % (SUB / 4);
-      const int ib = (q + d) * SUB, i0 = ib + 4 * ti, j0 = q * SUB + 4 * tj;
-      double acc[4][4] = {};
-      // X(i, ib + k) for k below the tile's rows is P[ib + k][i]; the last four
-      // k straddle the diagonal of X
-      for (int k = 0; k < 4 * ti; k++) {
-        double xi[4], tv[4];
+    if (act) {
+      // X(ib.., jb..) = -X(ib.., ib..) T
+      mfma32(SUB, lane, [&](int i, int k) { return X(ib + i, ib + k); },
+             [&](int k, int j) { return T[q][k][j]; }, acc);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          xi[u] = P[ib + k][i0 + u];
-          tv[u] = T[q][k][4 * tj + u];
-        }
+      for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int w = 0; w < 4; w++) acc[u][w] += xi[u] * tv[w];
-      }
-#pragma unroll
-      for (int k = 4 * ti; k < 4 * ti + 4; k++) {
-        double xi[4], tv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int i = i0 + u, kk = ib + k;
-          xi[u] = (i > kk) ? P[kk][i] : ((i == kk) ? P[kk][BK] : 0.0);
-          tv[u] = T[q][k][4 * tj + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int w = 0; w < 4; w++) acc[u][w] += xi[u] * tv[w];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int w = 0; w < 4; w++) P[j0 + w][i0 + u] = -acc[u][w];
+        for (int v = 0; v < 4; v++)
+          P[jb + 16 * (t & 1) + (lane & 15)][ib + 16 * (t >> 1) + 4 * v + (lane >> 4)] = -acc[t][v];
     }
     __syncthreads();
   }
@@ -473,7 +464,9 @@ void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, cons
 template <int BK>
 static size_t potrf_smem()
 {
-  return (BK * (BK + 1) + (BK / SUB - 1) * SUB * (SUB + 1)) * sizeof(double);
+  // P, the reciprocals, and one 32 x 33 product block per off-diagonal pair
+  // of the first block diagonal (at most NB-1 pairs at a time)
+  return (BK * (BK + 1) + BK + (BK / SUB - 1) * SUB * (SUB + 1)) * sizeof(double);
 }
 
 template <int BK>

@@ -37,6 +37,72 @@ int find_root(std::vector<int>& anc, int i)
 
 }  // namespace
 
+// Update-matrix space with reuse.  U(s) (nr x nr lower triangle, packed by
+// columns: nr (nr+1)/2 doubles) is written at s's level
+// and read at its parent's level, then dead: allocate the update matrices of
+// a level before freeing those its supernodes consume (their children's), so
+// the two never share space, with best fit over a list of free ranges.
+// u_off[s] is the offset of U(s) (unused for nr = 0), u_off[ns] the pool size.
+static void assign_update_space(CholPlan& pl)
+{
+  const int ns = pl.ns;
+  pl.u_off.assign(ns + 1, 0);
+  std::vector<std::pair<int64_t, int64_t>> freel;   // (offset, size), sorted by offset
+  int64_t top = 0;
+  auto alloc = [&](int64_t sz) -> int64_t {
+    int best = -1;
+    for (int i = 0; i < static_cast<int>(freel.size()); i++)
+      if (freel[i].second >= sz && (best < 0 || freel[i].second < freel[best].second)) best = i;
+    if (best < 0) {
+      // grow the pool, absorbing a free range that ends at the top
+      if (!freel.empty() && freel.back().first + freel.back().second == top) {
+        const int64_t off = freel.back().first;
+        freel.pop_back();
+        top = off + sz;
+        return off;
+      }
+      const int64_t off = top;
+      top += sz;
+      return off;
+    }
+    const int64_t off = freel[best].first;
+    freel[best].first += sz;
+    freel[best].second -= sz;
+    if (freel[best].second == 0) freel.erase(freel.begin() + best);
+    return off;
+  };
+  auto release = [&](int64_t off, int64_t sz) {
+    auto it = std::lower_bound(freel.begin(), freel.end(), std::make_pair(off, int64_t{0}));
+    it = freel.insert(it, {off, sz});
+    // coalesce with the next and the previous range
+    if (it + 1 != freel.end() && it->first + it->second == (it + 1)->first) {
+      it->second += (it + 1)->second;
+      freel.erase(it + 1);
+    }
+    if (it != freel.begin() && (it - 1)->first + (it - 1)->second == it->first) {
+      (it - 1)->second += it->second;
+      freel.erase(it);
+    }
+  };
+  const int nlev = static_cast<int>(pl.level_ptr.size()) - 1;
+  for (int l = 0; l < nlev; l++) {
+    for (int q = pl.level_ptr[l]; q < pl.level_ptr[l + 1]; q++) {
+      const int s = pl.level_sn[q];
+      const int64_t nr = pl.nr(s);
+      if (nr > 0) pl.u_off[s] = alloc(nr * (nr + 1) / 2);
+    }
+    for (int q = pl.level_ptr[l]; q < pl.level_ptr[l + 1]; q++) {
+      const int s = pl.level_sn[q];
+      for (int t = pl.child_ptr[s]; t < pl.child_ptr[s + 1]; t++) {
+        const int c = pl.child_list[t];
+        const int64_t nr = pl.nr(c);
+        if (nr > 0) release(pl.u_off[c], nr * (nr + 1) / 2);
+      }
+    }
+  }
+  pl.u_off[ns] = top;
+}
+
 CholPlan chol_analyze(const Pattern& a, const CholOptions& opt)
 {
   const int n = a.n;
@@ -265,16 +331,15 @@ CholPlan chol_analyze(const Pattern& a, const CholOptions& opt)
     for (int s = 0; s < ns; s++) pl.level_sn[f[pl.level[s]]++] = s;
   }
   pl.l_off.assign(ns + 1, 0);
-  pl.u_off.assign(ns + 1, 0);
   pl.nnz_l_stored = 0;
   pl.max_front = 0;
   for (int s = 0; s < ns; s++) {
     int64_t nc = pl.nc(s), nr = pl.nr(s), f = nc + nr;
     pl.l_off[s + 1] = pl.l_off[s] + f * nc;
-    pl.u_off[s + 1] = pl.u_off[s] + nr * nr;
     pl.nnz_l_stored += nc * (nc + 1) / 2 + nc * nr;
     pl.max_front = std::max<int>(pl.max_front, static_cast<int>(f));
   }
+  assign_update_space(pl);
   pl.rel.resize(pl.rows.size());
   for (int s = 0; s < ns; s++) {
     int p = pl.sn_parent[s];

@@ -97,6 +97,12 @@ void recover_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, in
                double* dsv, double* dyd, hipStream_t st);
 
 // ---------------------------------------------------------------- Cholesky
+// Offset of (r, c), r >= c, in an n x n lower triangle packed by columns.
+__host__ __device__ inline int64_t packed_lower(int64_t r, int64_t c, int64_t n)
+{
+  return c * n - c * (c + 1) / 2 + r;
+}
+
 struct CholDev {
   int n, ns;
   const int* sn_first;      // ns+1
@@ -104,7 +110,7 @@ struct CholDev {
   const int* rows;
   const int* rel;
   const int64_t* l_off;     // ns+1
-  const int64_t* u_off;     // ns+1
+  const int64_t* u_off;     // ns+1: U(s) offsets in the reused update space
   const int* child_ptr;
   const int* child_list;
   const int* sn_parent;

@@ -3,7 +3,8 @@
 // Storage (see CholPlan): supernode s owns columns [a, a+nc) and the sorted
 // off-diagonal-block rows `rows(s)` (nr of them); its front index list is
 // [cols; rows] (f = nc + nr).  The factor panel is f x nc column-major at
-// L + l_off[s]; the update matrix is nr x nr column-major at U + u_off[s].
+// L + l_off[s]; the update matrix is the nr x nr lower triangle packed by
+// columns at U + u_off[s].
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -50,13 +51,13 @@ __global__ void __launch_bounds__(256) k_extend_add(CholDev c, const int* __rest
     base = ip.Ls + static_cast<int64_t>(col) * ip.f;
     shift = 0;
   } else {
-    base = c.U + c.u_off[p] + static_cast<int64_t>(col - ip.nc) * ip.nr;
+    base = c.U + c.u_off[p] + packed_lower(0, col - ip.nc, ip.nr);   // rows >= col only
     shift = -ip.nc;
   }
   for (int k = cptr[t]; k < cptr[t + 1]; k++) {
     const int ch = cc[k], r2 = ccol[k];
     const int nrc = c.rows_ptr[ch + 1] - c.rows_ptr[ch];
-    const double* src = c.U + c.u_off[ch] + static_cast<int64_t>(r2) * nrc;
+    const double* src = c.U + c.u_off[ch] + packed_lower(0, r2, nrc);   // rows r1 >= r2
     const int* relc = c.rel + c.rows_ptr[ch];
     for (int r1 = r2 + threadIdx.x; r1 < nrc; r1 += blockDim.x) base[relc[r1] + shift] += src[r1];
     __syncthreads();

@@ -35,6 +35,9 @@ struct GemmArgs {
   int64_t ldc;
   int M, N, K;       // C is M x N (tile-local bounds), K inner
   double alpha, beta;
+  // pk > 0: C is the pk x pk lower triangle packed by columns and the tile
+  // sits at (pr0, pc0) in it; entries above the diagonal are not stored
+  int64_t pk = 0, pr0 = 0, pc0 = 0;
 };
 
 // One 128x128 tile of C = beta*C + alpha*A*B^T with beta in {0, 1} and
@@ -108,6 +111,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
     }
   }
+  if (g.pk > 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = wr + i * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = wc + j * 16 + 4 * v + (lane >> 4);
+          const int64_t gr = g.pr0 + r, gc = g.pc0 + c;
+          if (r < g.M && c < g.N && gr >= gc) g.C[packed_lower(gr, gc, g.pk)] = acc[i][j][v];
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = wr + i * 16 + (lane & 15);
@@ -154,8 +172,11 @@ __global__ void __launch_bounds__(256) k_dense_nt(CholDev c, const int* __restri
     g.A = L21 + R0;
     g.B = L21 + C0;
     g.lda = g.ldb = si.f;
-    g.C = c.U + c.u_off[tsn[blockIdx.x]] + static_cast<int64_t>(C0) * si.nr + R0;
+    g.C = c.U + c.u_off[tsn[blockIdx.x]];   // packed lower triangle of U
     g.ldc = si.nr;
+    g.pk = si.nr;
+    g.pr0 = R0;
+    g.pc0 = C0;
     g.M = min(BT, si.nr - R0);
     g.N = min(BT, si.nr - C0);
     g.K = si.nc;

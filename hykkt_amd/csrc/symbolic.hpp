@@ -58,7 +58,8 @@ void perm_map_r(const Pattern& a, const std::vector<int>& perm, Pattern& pa,
 // lower triangle.  Internal indices are a postorder of the elimination tree.
 // Front of supernode s: index list [cols(s); rows(s)] (f = nc + nr entries).
 // Factor panel: f x nc column-major at l_off[s] (lower part used).
-// Update matrix: nr x nr column-major at u_off[s] (lower part used).
+// Update matrix: nr x nr lower triangle at u_off[s], packed by columns
+// (column c holds rows c..nr-1 at offset c*nr - c(c-1)/2).
 struct CholPlan {
   int n = 0;
   int ns = 0;
@@ -73,7 +74,7 @@ struct CholPlan {
   std::vector<int> child_ptr;   // ns+1
   std::vector<int> child_list;  // children in increasing order
   std::vector<int64_t> l_off;   // ns+1 (doubles)
-  std::vector<int64_t> u_off;   // ns+1 (doubles)
+  std::vector<int64_t> u_off;   // ns+1 (doubles): U(s) offsets (reused space), [ns] = pool size
   std::vector<int> level;       // height in the supernodal tree (leaves 0)
   std::vector<int> level_ptr;   // nlevels+1
   std::vector<int> level_sn;    // supernodes grouped by level

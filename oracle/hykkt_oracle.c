@@ -1366,6 +1366,13 @@ int orc_execute(const orc_opts* o,
     jtpv[q] = jtv[jtmap[q]];
   }
 
+  /* add_diag(delta) (:392): the reference passes zero; a caller's delta
+     regularizes the factorized matrix, H_gamma + delta I in scaled terms */
+  if (o->delta != 0.0)
+    for (int i = 0; i < n; i++)
+      for (int q = hgprp[i]; q < hgprp[i + 1]; q++)
+        if (hgpci[q] == i) hgpv[q] += o->delta;
+
   /* ---- compute_hgamma_factorization (:398-401) ---- */
   int status = 0;
   int piv = orc_chol_factor(ch, hgpv, o->pivot_tol);

@@ -100,6 +100,8 @@ typedef struct {
   int    chol_kind;   /* orc_chol_create_kind() kind */
   int    refine_max;  /* iterative-refinement passes at most (0: none, as the reference) */
   double refine_tol;  /* refine while the 4-row residual is above this */
+  double delta;       /* added to the diagonal of the scaled, permuted H_gamma
+                         before factorization (add_diag hook, HykktSolver.cpp:392) */
 } orc_opts;
 
 typedef struct {

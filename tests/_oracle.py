@@ -83,7 +83,8 @@ def lib():
 class OrcOpts(C.Structure):
     _fields_ = [("gamma", C.c_double), ("ruiz_its", C.c_int), ("cg_tol", C.c_double),
                 ("cg_itmax", C.c_int), ("pivot_tol", C.c_double), ("norm_tol", C.c_double),
-                ("chol_kind", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double)]
+                ("chol_kind", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double),
+                ("delta", C.c_double)]
 
 
 class OrcStats(C.Structure):
@@ -277,14 +278,15 @@ def threads():
 
 def execute(H: Csr, Ds_vals, Jc: Csr, Jd: Csr, rx, rs, ry, ryd, gamma, perm=None, y0=None,
             ruiz_its=2, cg_tol=1e-12, cg_itmax=100, pivot_tol=1e-12, norm_tol=1e-2,
-            intermediates=False, chol_kind=0, refine_max=3, refine_tol=1e-9):
+            intermediates=False, chol_kind=0, refine_max=3, refine_tol=1e-9, delta=0.0):
     """Oracle execute(); returns dict with dx, ds, dyc, dyd, stats (+ intermediates).
     chol_kind 1 selects the supernodal Cholesky (same factor to rounding);
-    refine_max 0 stops after the reference's single pass."""
+    refine_max 0 stops after the reference's single pass; delta is added to
+    the scaled, permuted H_gamma's diagonal before factorization."""
     L = lib()
     n, m, d = H.n, Jc.n, len(Ds_vals)
     o = OrcOpts(gamma, ruiz_its, cg_tol, cg_itmax, pivot_tol, norm_tol, chol_kind, refine_max,
-                refine_tol)
+                refine_tol, delta)
     st = OrcStats()
     dx, dsv, dyc, dyd = np.zeros(n), np.zeros(d), np.zeros(m), np.zeros(d)
     pm = None if perm is None else np.ascontiguousarray(perm, np.int32)

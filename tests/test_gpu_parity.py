@@ -351,3 +351,43 @@ def test_iterative_refinement_gpu():
     x = np.concatenate(s.solution())
     xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
     assert rel(x, xo) < 1e-8
+
+
+def test_delta_hook_gpu():
+    """add_diag(delta) on the device: the single pass with delta solves the
+    KKT system whose H is H + delta diag(max_d)^-2 (the scaled diagonal
+    shift in the caller's variables) and matches the oracle run with the same
+    delta; with refinement the unshifted system's resid4 comes back below
+    1e-9 for a small delta; delta = 0 again gives the unshifted result bit for
+    bit."""
+    from test_oracle import dense_kkt
+    g = GridPattern(50, 65, seed=5, gamma=1e4, ds_range=(1e-4, 1e4))
+    sy = g.system()
+    Hc, Jc, Jd = to_csr(sy.H), to_csr(sy.Jc), to_csr(sy.Jd)
+    s = H.HykktSolver(sy.gamma)
+    bind(s, Hc, sy.Ds.v, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd)
+    assert s.execute() == 0
+    x_plain = np.concatenate(s.solution())
+    rhs = np.concatenate([sy.rx, sy.rs, sy.ry, sy.ryd])
+    K = dense_kkt(Hc, sy.Ds.v, Jc, Jd)
+    delta = 1e-6
+    s.options(delta=delta, refine_max=0)
+    s.execute()
+    st = s.stats()
+    x = np.concatenate(s.solution())
+    md = s.debug(4, st["n"] + st["m"])[:Hc.n]
+    ref = O.execute(Hc, sy.Ds.v, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd, sy.gamma,
+                    perm=s.permutation(), intermediates=True, refine_max=0, delta=delta)
+    assert np.array_equal(md, ref["max_d"][:Hc.n])
+    assert rel(x, np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])) < 1e-8
+    Kd = K.copy()
+    Kd[np.arange(Hc.n), np.arange(Hc.n)] += delta / md ** 2
+    assert np.linalg.norm(Kd @ x - rhs) / np.linalg.norm(rhs) <= 1e-9
+    assert st["resid4"] > 1e-6   # the unshifted system is not what was solved
+    s.options(delta=1e-8, refine_max=3)
+    s.execute()
+    st = s.stats()
+    assert st["refine_steps"] >= 1 and st["resid4"] <= 1e-9
+    s.options(delta=0.0)
+    assert s.execute() == 0
+    assert np.array_equal(np.concatenate(s.solution()), x_plain)

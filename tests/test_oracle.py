@@ -384,3 +384,34 @@ def test_execute_iterative_refinement():
     assert np.linalg.norm(K @ mine - rhs) / np.linalg.norm(rhs) <= 1e-10
     single = np.concatenate([one["dx"], one["ds"], one["dyc"], one["dyd"]])
     assert np.linalg.norm(mine - sol) < np.linalg.norm(single - sol)
+
+
+def test_delta_hook_solves_shifted_system():
+    """add_diag(delta) adds delta to the diagonal of the Ruiz-scaled, permuted
+    H_gamma.  In the caller's variables x = D x^ (D = diag(max_d[:n])) that is
+    H + delta D^-2, so the single pass solves the dense 4x4 KKT with that H to
+    the usual accuracy, while its residual against the unshifted system grows
+    with delta; refinement against the unshifted system recovers it for a
+    small delta.  delta = 0 is bit for bit the default."""
+    g = GridPattern(50, 65, seed=5, gamma=1e4, ds_range=(1e-4, 1e4))
+    s = g.system()
+    c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
+    H, Jc, Jd = c(s.H), c(s.Jc), c(s.Jd)
+    args = (H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma)
+    base = O.execute(*args)
+    zero = O.execute(*args, delta=0.0)
+    assert np.array_equal(base["dx"], zero["dx"]) and np.array_equal(base["dyc"], zero["dyc"])
+    rhs = np.concatenate([s.rx, s.rs, s.ry, s.ryd])
+    for delta in (1e-8, 1e-4):
+        one = O.execute(*args, delta=delta, refine_max=0, intermediates=True)
+        md = one["max_d"][:H.n]
+        K = dense_kkt(H, s.Ds.v, Jc, Jd)
+        Kd = K.copy()
+        Kd[np.arange(H.n), np.arange(H.n)] += delta / md ** 2
+        x = np.concatenate([one["dx"], one["ds"], one["dyc"], one["dyd"]])
+        shifted = np.linalg.norm(Kd @ x - rhs) / np.linalg.norm(rhs)
+        plain = np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs)
+        assert shifted <= 1e-9 and plain > 1e3 * shifted, (delta, shifted, plain)
+        assert np.linalg.norm(x - np.linalg.solve(Kd, rhs)) <= 1e-6 * np.linalg.norm(x)
+    refined = O.execute(*args, delta=1e-8)
+    assert refined["stats"]["refine_steps"] >= 1 and refined["stats"]["resid4"] <= 1e-9

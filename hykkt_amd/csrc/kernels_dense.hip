@@ -158,7 +158,7 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 // mode 0: trailing panel update after block `step`   C = C - A A^T (K = block)
 // mode 1: update matrix U = -L21 L21^T                 (K = nc, U overwritten)
 // mode 2: TRSM L21(block) = A21 * Dinv^T               (K = block, in place)
-__global__ void __launch_bounds__(256) k_dense_nt(CholDev c, const int* __restrict__ tsn,
+__global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __restrict__ tsn,
                                                   const int* __restrict__ ttr,
                                                   const int* __restrict__ ttc, int mode, int step)
 {

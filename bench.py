@@ -172,30 +172,43 @@ def main():
               f"cg={st['t_cg_ms']:.1f}ms sym={st['t_symbolic_ms']:.0f}ms", file=sys.stderr,
               flush=True)
 
-    lat, worst, iters, tf, tcg, nref, first = [], 0.0, [], [], [], [], 0.0
-    if args.kernel_timing:
-        s.profile(True)   # HIP events around every factor / solve launch
-    H.device_synchronize()
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        bind(args.warmup + k)
-        t0 = time.perf_counter()
-        rc = s.execute()
-        lat.append(time.perf_counter() - t0)
-        st = s.stats()
-        worst = max(worst, st["resid4"])
-        iters.append(st["cg_iters"])
-        tf.append(st["t_factor_ms"])
-        tcg.append(st["t_cg_ms"])
-        nref.append(st["refine_steps"])
-        first = max(first, st["resid4_first"])
-        if rc != 0 or not st["resid4"] <= 1e-8:
-            print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
-                  flush=True)
-    H.device_synchronize()
-    total = time.perf_counter() - t_start
-    kt = s.kernel_times() if args.kernel_timing else {}
-    s.profile(False)
+    def timed_pass(profile):
+        """K executes over the next systems of the sequence between two device
+        synchronizations; with `profile`, HIP events bracket every factor and
+        solve launch (a few microseconds each, so this pass runs slower)."""
+        r = dict(lat=[], worst=0.0, iters=[], tf=[], tcg=[], nref=[], first=0.0)
+        if profile:
+            s.profile(True)
+        H.device_synchronize()
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            bind(args.warmup + k)
+            t0 = time.perf_counter()
+            rc = s.execute()
+            r["lat"].append(time.perf_counter() - t0)
+            st = s.stats()
+            r["worst"] = max(r["worst"], st["resid4"])
+            r["iters"].append(st["cg_iters"])
+            r["tf"].append(st["t_factor_ms"])
+            r["tcg"].append(st["t_cg_ms"])
+            r["nref"].append(st["refine_steps"])
+            r["first"] = max(r["first"], st["resid4_first"])
+            if rc != 0 or not st["resid4"] <= 1e-8:
+                print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
+                      flush=True)
+        H.device_synchronize()
+        r["total"] = time.perf_counter() - t_start
+        r["kt"] = s.kernel_times() if profile else {}
+        s.profile(False)
+        return r
+
+    # the measured pass runs the plain path with nothing added; the per-kernel
+    # figures come from a second pass over the same systems with event timing
+    main_pass = timed_pass(False)
+    prof_pass = timed_pass(True) if args.kernel_timing else None
+    kt = prof_pass["kt"] if prof_pass else {}
+    lat, worst, iters, tf, tcg, nref, first, total = (
+        main_pass[k] for k in ("lat", "worst", "iters", "tf", "tcg", "nref", "first", "total"))
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": metric_name(),
@@ -223,6 +236,11 @@ def main():
         "refine_steps_mean": float(np.mean(nref)),
         "max_resid4_before_refinement": first,
         "roofline": roofline(kt, args.steps),
+        "kernel_timing_pass": None if prof_pass is None else {
+            "ms_per_step": 1e3 * prof_pass["total"] / args.steps,
+            "max_resid4": prof_pass["worst"],
+            "note": "same K executes again with HIP events around every factor and solve "
+                    "launch; roofline and kernels come from this pass"},
         "kernels": {k: {"launches_per_step": v["launches"] / args.steps,
                         "ms_per_step": v["ms"] / args.steps,
                         "avg_us": 1e3 * v["ms"] / max(v["launches"], 1),

@@ -220,24 +220,26 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
 
 // Factor the diagonal block of step `step` (b <= 128) in LDS, write L_kk back
 // and its inverse (lower triangular, zero-padded to 128 x 128, column-major)
-// to dinv.  LDS holds one BK x (BK+1) array P: L(r, c) (r >= c) at P[r][c];
-// the inverse X(i, j) (i > j) at P[j][i] and X(j, j) at P[j][BK]; the
-// reciprocals of L's diagonal; and 32 x 33 scratch blocks for the inverse's
-// off-diagonal products (one per block pair of a block diagonal).  The block is padded to BK with the
-// identity.
+// to dinv.  LDS holds one BK x (BK+1) array P, padded to BK with the
+// identity: L(r, c) (r >= c) at P[r][c], the inverse X(i, j) (i > j) at
+// P[j][i] and X(j, j) at P[j][BK]; and the reciprocals of L's diagonal.
 //
-// Both steps are blocked by 32.  Cholesky, per 32-column panel: wave 0
-// factors the 32x32 diagonal block in registers (lane r holds row r; column
-// k reaches all lanes through v_readlane), every thread solves one row below
-// against it (multiplying by reciprocals, no divisions on the chain), and the
-// trailing lower part is updated with FP64 MFMA 16x16x4 tiles.  Inverse: wave
-// w inverts diagonal block w (lane j computes column j), then the block
-// diagonals below follow from X(i,j) = -X(i,i) * sum_{k=j}^{i-1} L(i,k) X(k,j),
-// both products on the matrix cores.
+// Cholesky, blocked by 16-column panels: wave 0 factors the panel's 16x16
+// diagonal block in registers (lane r holds row r; column k reaches every
+// lane through v_readlane; 1/sqrt of the pivot is v_rsq_f64 refined by two
+// Newton steps, so the chain carries no division or square root); one thread
+// per row below solves that row against the block (multiplying by the
+// reciprocals); the trailing lower part takes the rank-16 update as FP64 MFMA
+// 16x16x4 tiles spread over the four waves.  Inverse: the 16x16 diagonal
+// blocks are inverted together (16 lanes per block, lane j computes column
+// j), then blocks of 16, 32 and 64 are doubled with
+//   inv([L11 0; L21 L22]) = [X11 0; -X22 (L21 X11)  X22],
+// both products on the matrix cores; L21 X11 is parked in the slot that
+// -X22 (L21 X11) then overwrites.
 //
 // MFMA f64 16x16x4 (lane l of 64): A operand A(l%16, l/16), B operand
 // B(l/16, l%16), result element v = D(4v + l/16, l%16) of D = A B.
-constexpr int SUB = 32;
+constexpr int PW = 16;   // panel width
 
 // Lane `lane`'s value of v, as a wave-uniform (scalar register) value.
 __device__ __forceinline__ double readlane_f64(double v, int lane)
@@ -248,35 +250,81 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// One 16x16 tile of D = A B^T-style products accumulated over K (a multiple
-// of 4): a_at(i, k) and b_at(k, j) give the operands.
-template <class FA, class FB>
-__device__ __forceinline__ d4 mfma16(int K, int lane, FA a_at, FB b_at)
+// One 16x16 tile of D = A B accumulated over K (a multiple of 4): a_at(i, k)
+// and b_at(k, j) give the operands.
+template <int K, class FA, class FB>
+__device__ __forceinline__ d4 mfma16(int lane, FA a_at, FB b_at)
 {
   d4 acc = {0.0, 0.0, 0.0, 0.0};
   const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
   for (int k0 = 0; k0 < K; k0 += 4)
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a_at(i, k0 + kq), b_at(k0 + kq, i), acc, 0, 0, 0);
   return acc;
 }
 
-// A 32x32 block of D = A B as four 16x16 tiles with independent accumulators
-// (interleaved, so the matrix core pipelines them): acc[2*ti + tj] holds
-// tile (ti, tj), element v at D(16 ti + 4v + lane/16, 16 tj + lane%16).
-template <class FA, class FB>
-__device__ __forceinline__ void mfma32(int K, int lane, FA a_at, FB b_at, d4 (&acc)[4])
+// X(i, j) of the inverse from its storage in P (0 above the diagonal).
+template <int BK>
+__device__ __forceinline__ double xinv(const double (*P)[BK + 1], int i, int j)
 {
+  return (i > j) ? P[j][i] : ((i == j) ? P[j][BK] : 0.0);
+}
+
+// One doubling level: for every pair of inverted diagonal blocks of size H,
+// X21 = -X22 (L21 X11).  Tile t of NT = pairs * (H/16)^2 goes to wave t % 4.
+template <int BK, int H>
+__device__ __forceinline__ void inverse_level(double (*P)[BK + 1], int wave, int lane)
+{
+  constexpr int HB = H / 16, TPP = HB * HB, NT = (BK / (2 * H)) * TPP, PERW = (NT + 3) / 4;
+  int r0[PERW], c0[PERW], b1[PERW];
 #pragma unroll
-  for (int t = 0; t < 4; t++) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-  const int i = lane & 15, kq = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const double a0 = a_at(i, k0 + kq), a1 = a_at(16 + i, k0 + kq);
-    const double b0 = b_at(k0 + kq, i), b1 = b_at(k0 + kq, 16 + i);
-    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[1], 0, 0, 0);
-    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[2], 0, 0, 0);
-    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[3], 0, 0, 0);
+  for (int u = 0; u < PERW; u++) {
+    const int t = wave + 4 * u, q = t / TPP, tt = t - q * TPP;
+    b1[u] = 2 * q * H;
+    r0[u] = b1[u] + H + 16 * (tt / HB);
+    c0[u] = b1[u] + 16 * (tt % HB);
   }
+  const int i = lane & 15, kq = lane >> 4;
+  // T = L21 X11, stored where X21 goes (T(r, c) at P[c][r])
+  d4 acc[PERW];
+#pragma unroll
+  for (int u = 0; u < PERW; u++) {
+    acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+    if (wave + 4 * u < NT) {
+#pragma unroll
+      for (int k0 = 0; k0 < H; k0 += 4) {
+        const int k = b1[u] + k0 + kq;
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[r0[u] + i][k], xinv<BK>(P, k, c0[u] + i),
+                                                      acc[u], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PERW; u++)
+    if (wave + 4 * u < NT)
+#pragma unroll
+      for (int v = 0; v < 4; v++) P[c0[u] + i][r0[u] + 4 * v + kq] = acc[u][v];
+  __syncthreads();
+  // X21 = -X22 T; every read of T before the in-place writes
+#pragma unroll
+  for (int u = 0; u < PERW; u++) {
+    acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+    if (wave + 4 * u < NT) {
+#pragma unroll
+      for (int k0 = 0; k0 < H; k0 += 4) {
+        const int k = b1[u] + H + k0 + kq;
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xinv<BK>(P, r0[u] + i, k), P[c0[u] + i][k],
+                                                      acc[u], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PERW; u++)
+    if (wave + 4 * u < NT)
+#pragma unroll
+      for (int v = 0; v < 4; v++) P[c0[u] + i][r0[u] + 4 * v + kq] = -acc[u][v];
+  __syncthreads();
 }
 
 // BK: padded block edge (32, 64 or 128) chosen per launch from the largest
@@ -285,7 +333,7 @@ template <int BK>
 __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
                                                    int step)
 {
-  constexpr int NB = BK / SUB;   // 32-wide panels
+  constexpr int NPN = BK / PW;   // panels
   extern __shared__ double smem[];
   const int s = sns[blockIdx.x];
   const SnInfo si = sninfo(c, s);
@@ -293,8 +341,7 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   if (k0 >= si.nc) return;
   const int b = min(BK, si.nc - k0);
   double (*P)[BK + 1] = reinterpret_cast<double (*)[BK + 1]>(smem);
-  double* rd = smem + BK * (BK + 1);                                   // 1 / L(r, r)
-  double (*T)[SUB][SUB + 1] = reinterpret_cast<double (*)[SUB][SUB + 1]>(rd + BK);
+  double* rd = smem + BK * (BK + 1);   // 1 / L(r, r)
   __shared__ int bad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
@@ -317,28 +364,24 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   }
   if (tid == 0) bad = INT_MAX;
   __syncthreads();
-
-  // X(i, j) for i >= j from its storage; 0 above the diagonal
-  auto X = [&](int i, int j) -> double { return (i > j) ? P[j][i] : ((i == j) ? P[j][BK] : 0.0); };
   const double tol2 = c.pivot_tol * c.pivot_tol;
 
-  // ---- blocked Cholesky; the diagonal blocks of the inverse on the way ----
+  // ---- blocked Cholesky ----
 #pragma unroll 1
-  for (int p = 0; p < NB; p++) {
-    const int kb = p * SUB;
+  for (int p = 0; p < NPN; p++) {
+    const int kb = p * PW;
     if (wave == 0) {
-      // lane r holds row r of the diagonal block; entries above the diagonal
-      // are carried along (never stored), so no lane-dependent branches.
-      // Step k: pivot from lane k, 1/sqrt from v_rsq_f64 refined by two
-      // Newton steps, column k to every lane through v_readlane.
-      const int r = lane & (SUB - 1);
-      double v[SUB];
+      // lane r holds row r of the diagonal block (lanes >= 16 repeat rows
+      // 0..15 and store nothing); entries above the diagonal are carried
+      // along, never stored, so no lane-dependent branches.
+      const int r = lane & (PW - 1);
+      double v[PW];
 #pragma unroll
-      for (int q = 0; q < SUB; q++) v[q] = P[kb + r][kb + q];
+      for (int q = 0; q < PW; q++) v[q] = P[kb + r][kb + q];
       int badk = -1;
       double myinv = 1.0;   // 1 / L(r, r), taken at step k == r (no dynamic register index)
 #pragma unroll
-      for (int k = 0; k < SUB; k++) {
+      for (int k = 0; k < PW; k++) {
         const double dkk = readlane_f64(v[k], k);
         const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
         badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
@@ -351,78 +394,48 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
         myinv = (r == k) ? y : myinv;
         const double vk = v[k];
 #pragma unroll
-        for (int q = k + 1; q < SUB; q++) v[q] -= vk * readlane_f64(vk, q);
+        for (int q = k + 1; q < PW; q++) v[q] -= vk * readlane_f64(vk, q);
       }
       if (lane == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
-      if (lane < SUB) {
+      if (lane < PW) {
 #pragma unroll
-        for (int q = 0; q < SUB; q++)
+        for (int q = 0; q < PW; q++)
           if (q <= r) P[kb + r][kb + q] = v[q];
         rd[kb + r] = myinv;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // inverse of this diagonal block: lane j < 32 computes column j;
-      // X(i, j) (i > j) goes to P[j][i], X(j, j) to P[j][BK]
-      if (lane < SUB) {
-        const int j = lane;
-        double x[SUB];
-#pragma unroll
-        for (int i = 0; i < SUB; i++) {
-          double t0 = (i == j) ? 1.0 : 0.0, t1 = 0.0;
-#pragma unroll
-          for (int k = 0; k < i; k++) {
-            if (k & 1) t1 -= P[kb + i][kb + k] * x[k];
-            else t0 -= P[kb + i][kb + k] * x[k];
-          }
-          x[i] = (i >= j) ? (t0 + t1) * rd[kb + i] : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < SUB; i++) {
-          if (i == j) P[kb + j][BK] = x[i];
-          else if (i > j) P[kb + j][kb + i] = x[i];
-        }
-      }
     }
     __syncthreads();
-    const int rbeg = kb + SUB;
+    const int rbeg = kb + PW;
     if (rbeg < BK) {
-      // rows below: L(r, panel) = A(r, panel) X_pp^T on the matrix cores,
-      // 16-row x 16-column tiles; all reads before the in-place writes
-      const int nrt = (BK - rbeg) / 16, ntl = 2 * nrt;
-      d4 acc[3];
+      // rows below: thread t solves row rbeg + t against the diagonal block
+      if (tid < BK - rbeg) {
+        const int r = rbeg + tid;
+        double x[PW];
 #pragma unroll
-      for (int u = 0; u < 3; u++) {
-        const int t = wave + 4 * u;
-        if (t < ntl) {
-          const int r0 = rbeg + 16 * (t >> 1), c0 = 16 * (t & 1);
-          acc[u] = mfma16(SUB, lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
-                          [&](int k, int j) { return X(kb + c0 + j, kb + k); });
+        for (int j = 0; j < PW; j++) {
+          double t0 = P[r][kb + j], t1 = 0.0;
+#pragma unroll
+          for (int k = 0; k < j; k++) {
+            if (k & 1) t1 -= x[k] * P[kb + j][kb + k];
+            else t0 -= x[k] * P[kb + j][kb + k];
+          }
+          x[j] = (t0 + t1) * rd[kb + j];
         }
-      }
-      __syncthreads();
 #pragma unroll
-      for (int u = 0; u < 3; u++) {
-        const int t = wave + 4 * u;
-        if (t < ntl) {
-          const int r0 = rbeg + 16 * (t >> 1), c0 = 16 * (t & 1);
-#pragma unroll
-          for (int v = 0; v < 4; v++) P[r0 + 4 * v + (lane >> 4)][kb + c0 + (lane & 15)] = acc[u][v];
-        }
+        for (int j = 0; j < PW; j++) P[r][kb + j] = x[j];
       }
       __syncthreads();
       // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k] over the panel,
-      // r >= cc >= rbeg, in 16x16 MFMA tiles (lower tiles only; the upper
-      // half of diagonal tiles is scratch)
-      const int mt = (BK - rbeg) / 16;
-      for (int e = wave; e < mt * (mt + 1) / 2; e += 4) {
+      // r >= cc >= rbeg, in 16x16 tiles (lower tiles only; the upper half of
+      // a diagonal tile is scratch until the inverse is written)
+      const int nrt = (BK - rbeg) / PW;
+      for (int e = wave; e < nrt * (nrt + 1) / 2; e += 4) {
         int ti = 0;
         while ((ti + 1) * (ti + 2) / 2 <= e) ti++;
         const int tj = e - ti * (ti + 1) / 2;
-        const int r0 = rbeg + 16 * ti, c0 = rbeg + 16 * tj;
-        const d4 a = mfma16(SUB, lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
-                            [&](int k, int j) { return P[c0 + j][kb + k]; });
+        const int r0 = rbeg + PW * ti, c0 = rbeg + PW * tj;
+        const d4 a = mfma16<PW>(lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
+                                [&](int k, int j) { return P[c0 + j][kb + k]; });
 #pragma unroll
         for (int v = 0; v < 4; v++) P[r0 + 4 * v + (lane >> 4)][c0 + (lane & 15)] -= a[v];
       }
@@ -430,36 +443,30 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
     }
   }
 
-  // block diagonals d = 1 .. NB-1: pair q (rows ib = (q+d)*32, cols jb = q*32) on wave q
-#pragma unroll 1
-  for (int d = 1; d < NB; d++) {
-    const int q = wave;
-    const bool act = q < NB - d;
-    const int ib = (q + d) * SUB, jb = q * SUB;
-    d4 acc[4];
-    if (act) {
-      // T = sum_{k = jb}^{ib-1} L(ib.., k) X(k, jb..)   (32 x 32, K = 32 d)
-      mfma32(SUB * d, lane, [&](int i, int k) { return P[ib + i][jb + k]; },
-             [&](int k, int j) { return X(jb + k, jb + j); }, acc);
+  // ---- inverse: the 16x16 diagonal blocks (16 lanes each), then doubling ----
+  if (tid < BK) {
+    const int bb = (tid / PW) * PW, j = tid % PW;
+    double x[PW];
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+    for (int i = 0; i < PW; i++) {
+      double t0 = (i == j) ? 1.0 : 0.0, t1 = 0.0;
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-          T[q][16 * (t >> 1) + 4 * v + (lane >> 4)][16 * (t & 1) + (lane & 15)] = acc[t][v];
+      for (int k = 0; k < i; k++) {
+        if (k & 1) t1 -= P[bb + i][bb + k] * x[k];
+        else t0 -= P[bb + i][bb + k] * x[k];
+      }
+      x[i] = (i >= j) ? (t0 + t1) * rd[bb + i] : 0.0;
     }
-    __syncthreads();
-    if (act) {
-      // X(ib.., jb..) = -X(ib.., ib..) T
-      mfma32(SUB, lane, [&](int i, int k) { return X(ib + i, ib + k); },
-             [&](int k, int j) { return T[q][k][j]; }, acc);
 #pragma unroll
-      for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          P[jb + 16 * (t & 1) + (lane & 15)][ib + 16 * (t >> 1) + 4 * v + (lane >> 4)] = -acc[t][v];
+    for (int i = 0; i < PW; i++) {
+      if (i == j) P[bb + j][BK] = x[i];
+      else if (i > j) P[bb + j][bb + i] = x[i];
     }
-    __syncthreads();
   }
+  __syncthreads();
+  if constexpr (BK >= 32) inverse_level<BK, 16>(P, wave, lane);
+  if constexpr (BK >= 64) inverse_level<BK, 32>(P, wave, lane);
+  if constexpr (BK >= 128) inverse_level<BK, 64>(P, wave, lane);
 
   for (int col = tid / 64; col < b; col += 4)
     for (int r = (tid & 63) + col; r < b; r += 64) D[r + static_cast<int64_t>(col) * si.f] = P[r][col];
@@ -467,7 +474,7 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   // b x b lower triangle stay at the zeros written when the slot was created
   double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
   for (int col = tid / 64; col < b; col += 4)
-    for (int r = (tid & 63) + col; r < b; r += 64) W[static_cast<int64_t>(col) * BT + r] = X(r, col);
+    for (int r = (tid & 63) + col; r < b; r += 64) W[static_cast<int64_t>(col) * BT + r] = xinv<BK>(P, r, col);
   if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
 }
 
@@ -485,9 +492,8 @@ void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, cons
 template <int BK>
 static size_t potrf_smem()
 {
-  // P, the reciprocals, and one 32 x 33 product block per off-diagonal pair
-  // of the first block diagonal (at most NB-1 pairs at a time)
-  return (BK * (BK + 1) + BK + (BK / SUB - 1) * SUB * (SUB + 1)) * sizeof(double);
+  // P and the reciprocals of the diagonal
+  return (BK * (BK + 1) + BK) * sizeof(double);
 }
 
 template <int BK>

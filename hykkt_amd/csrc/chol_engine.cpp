@@ -18,6 +18,11 @@ namespace hykkt {
 
 namespace {
 constexpr int NB = 128;   // panel step, MFMA tile edge, solve block
+constexpr int SR = 32;    // rows of a short tile
+// A step whose TRSM has at most this many 128-row tiles runs its TRSM and
+// next-column update in short tiles: four times the workgroups, each with a
+// quarter of the work, for launches that sit on the critical path.
+constexpr int SHORT_MAX = 128;
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
@@ -145,6 +150,15 @@ void CholEngine::build_tasks()
     for (int j = 0; j < nsteps; j++) {
       L.potrf[j].off = static_cast<int>(potrf.size());
       L.trsm[j].off = static_cast<int>(trsm_sn.size());
+      int full_tiles = 0;
+      for (int q = b0; q < b1; q++) {
+        const int s = pl_.level_sn[q];
+        const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
+        if (k0 < nc) full_tiles += cdiv(f - (k0 + std::min(NB, nc - k0)), NB);
+      }
+      const bool shrt = full_tiles <= SHORT_MAX;
+      const int rows_unit = shrt ? SR : NB;
+      L.trsm[j].short_rows = L.upd_next[j].short_rows = shrt;
       for (int q = b0; q < b1; q++) {
         const int s = pl_.level_sn[q];
         const int nc = pl_.nc(s), f = nc + pl_.nr(s);
@@ -153,7 +167,6 @@ void CholEngine::build_tasks()
         const int b = std::min(NB, nc - k0);
         potrf.push_back(s);
         L.potrf[j].maxb = std::max(L.potrf[j].maxb, b);
-        const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
         // Algorithmic (LAPACK) counts: POTRF b^3/3, TRSM r*b^2 for the r rows
         // below the block, and the lower-trapezoid rank-b update of the c
         // trailing panel columns; bytes read each operand and write each
@@ -172,8 +185,7 @@ void CholEngine::build_tasks()
         L.upd_next[j].bytes += 8.0 * (r * db + 2 * pn);
         L.upd_rest[j].flops += 2.0 * db * (pairs - pn);
         L.upd_rest[j].bytes += 8.0 * 2 * (pairs - pn);
-        (void)ctiles;
-        for (int t = 0; t < rtiles; t++) {
+        for (int t = 0; t < cdiv(f - (k0 + b), rows_unit); t++) {
           trsm_sn.push_back(s);
           trsm_tr.push_back(t);
           trsm_tc.push_back(0);
@@ -190,6 +202,15 @@ void CholEngine::build_tasks()
           if (k0 >= nc) continue;
           const int b = std::min(NB, nc - k0);
           const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
+          if (part == 0 && shrt) {   // next block column in 32-row tiles
+            if (ctiles > 0)
+              for (int tr = 0; tr < cdiv(f - (k0 + b), SR); tr++) {
+                upd_sn.push_back(s);
+                upd_tr.push_back(tr);
+                upd_tc.push_back(0);
+              }
+            continue;
+          }
           for (int tc = part == 0 ? 0 : 1; tc < (part == 0 ? std::min(1, ctiles) : ctiles); tc++)
             for (int tr = tc; tr < rtiles; tr++) {
               upd_sn.push_back(s);
@@ -373,14 +394,15 @@ void CholEngine::factor(double pivot_tol)
       });
       timed(PK_DENSE_NT, L.trsm[j], [&] {
         dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
-                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_);
+                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_,
+                      L.trsm[j].short_rows);
       });
       if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
       rest_pending = false;
       timed(PK_DENSE_NT, L.upd_next[j], [&] {
         dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
                       t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, 0, step,
-                      st_);
+                      st_, L.upd_next[j].short_rows);
       });
       if (L.upd_rest[j].cnt > 0) {
         HIPCHK(hipEventRecord(ev_main_, st_));

@@ -94,6 +94,7 @@ class CholEngine {
  private:
   struct Launch {
     int off = 0, cnt = 0, maxb = 0;
+    bool short_rows = false;   // dense_nt tiles of 32 rows (short critical-path launches)
     double flops = 0.0, bytes = 0.0;   // algorithmic, for the profiler
   };
   struct Level {

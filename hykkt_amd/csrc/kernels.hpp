@@ -132,9 +132,10 @@ void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task
 void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st);
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the
-// rows below block `step` with its inverted diagonal block.
+// rows below block `step` with its inverted diagonal block.  short_rows:
+// 32-row x 128-column tiles, tile rows counted in 32-row units (modes 0, 2).
 void dense_nt(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
-              const int* task_tc, int mode, int step, hipStream_t st);
+              const int* task_tc, int mode, int step, hipStream_t st, bool short_rows = false);
 
 // Single-launch supernodal solves over the whole tree.  Work items are
 // 128-row blocks of the fronts, dispatched in a dependency-respecting order;

@@ -40,27 +40,33 @@ struct GemmArgs {
   int64_t pk = 0, pr0 = 0, pc0 = 0;
 };
 
-// One 128x128 tile of C = beta*C + alpha*A*B^T with beta in {0, 1} and
-// alpha in {1, -1}.  256 threads: 4 waves in a 2x2 arrangement, each owning a
-// 64x64 quadrant = 4x4 MFMA blocks.  The accumulators start from beta*C (all
-// loads in flight at once) and are stored once at the end; alpha is applied
-// exactly to the staged A values.  K is staged through LDS in chunks of KC,
-// the next chunk's global loads overlapping the current chunk's MFMAs.
+// One TM x TN tile of C = beta*C + alpha*A*B^T with beta in {0, 1} and
+// alpha in {1, -1}.  256 threads: 4 waves in a WM x WN arrangement (2x2 for
+// 128x128 tiles, 1x4 for 32x128), each owning a QM x QN block of IM x JN
+// MFMA tiles.  The accumulators start from beta*C (all loads in flight at
+// once) and are stored once at the end; alpha is applied exactly to the
+// staged A values.  K is staged through LDS in chunks of KC, the next chunk's
+// global loads overlapping the current chunk's MFMAs.
+template <int TM, int TN>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 {
-  double* As = smem;                       // [KC][LDA_S]
-  double* Bs = smem + KC * LDA_S;          // [KC][LDA_S]
+  constexpr int WN = TM == TN ? 2 : 4, WM = 4 / WN;
+  constexpr int QM = TM / WM, QN = TN / WN, IM = QM / 16, JN = QN / 16;
+  constexpr int LA = TM + 2, LB = TN + 2;          // LDS row lengths (doubles)
+  constexpr int EA = TM / 16, EB = TN / 16;        // staged elements per thread
+  double* As = smem;                               // [KC][LA]
+  double* Bs = smem + KC * LA;                     // [KC][LB]
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  d4 acc[4][4];
+  const int wr = (wave / WN) * QM, wc = (wave % WN) * QN;
+  d4 acc[IM][JN];
   // acc[i][j][v]: C row = wr + 16 i + lane%16, C col = wc + 16 j + 4 v + lane/16
   if (g.beta != 0.0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < IM; i++) {
       const int r = wr + i * 16 + (lane & 15);
 #pragma unroll
-      for (int j = 0; j < 4; j++)
+      for (int j = 0; j < JN; j++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int c = wc + j * 16 + 4 * v + (lane >> 4);
@@ -69,20 +75,24 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < IM; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < JN; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   }
 
-  // staging assignment: thread t loads k = t/16, rows (t%16)*8 .. +8
-  const int sk = t >> 4, sr = (t & 15) * 8;
-  double ra[8], rb[8];
+  // staging assignment: thread t loads k = t/16, rows (t%16)*E .. +E
+  const int sk = t >> 4, sa = (t & 15) * EA, sb = (t & 15) * EB;
+  double ra[EA], rb[EB];
   auto gload = [&](int k0) {
     const int kk = k0 + sk;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int r = sr + q;
+    for (int q = 0; q < EA; q++) {
+      const int r = sa + q;
       ra[q] = (kk < g.K && r < g.M) ? g.A[kk * g.lda + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      const int r = sb + q;
       rb[q] = (kk < g.K && r < g.N) ? g.B[kk * g.ldb + r] : 0.0;
     }
   };
@@ -90,33 +100,32 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
   for (int k0 = 0; k0 < g.K; k0 += KC) {
     __syncthreads();   // the previous chunk's fragments have been read
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      As[sk * LDA_S + sr + q] = g.alpha * ra[q];   // exact: alpha is +-1
-      Bs[sk * LDA_S + sr + q] = rb[q];
-    }
+    for (int q = 0; q < EA; q++) As[sk * LA + sa + q] = g.alpha * ra[q];   // exact: alpha is +-1
+#pragma unroll
+    for (int q = 0; q < EB; q++) Bs[sk * LB + sb + q] = rb[q];
     __syncthreads();
     if (k0 + KC < g.K) gload(k0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC / 4; ks++) {
       const int kr = ks * 4 + (lane >> 4);
-      double fa[4], fb[4];
+      double fa[IM], fb[JN];
 #pragma unroll
-      for (int i = 0; i < 4; i++) fa[i] = As[kr * LDA_S + wr + i * 16 + (lane & 15)];
+      for (int i = 0; i < IM; i++) fa[i] = As[kr * LA + wr + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; j++) fb[j] = Bs[kr * LDA_S + wc + j * 16 + (lane & 15)];
+      for (int j = 0; j < JN; j++) fb[j] = Bs[kr * LB + wc + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < IM; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < JN; j++)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
     }
   }
   if (g.pk > 0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < IM; i++) {
       const int r = wr + i * 16 + (lane & 15);
 #pragma unroll
-      for (int j = 0; j < 4; j++)
+      for (int j = 0; j < JN; j++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int c = wc + j * 16 + 4 * v + (lane >> 4);
@@ -127,10 +136,10 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
     return;
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < IM; i++) {
     const int r = wr + i * 16 + (lane & 15);
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < JN; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int c = wc + j * 16 + 4 * v + (lane >> 4);
@@ -158,6 +167,11 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 // mode 0: trailing panel update after block `step`   C = C - A A^T (K = block)
 // mode 1: update matrix U = -L21 L21^T                 (K = nc, U overwritten)
 // mode 2: TRSM L21(block) = A21 * Dinv^T               (K = block, in place)
+// TM = 128: 128x128 tiles (all modes).  TM = 32: 32-row x 128-column tiles
+// (modes 0 and 2), four times as many workgroups for the short launches on
+// the critical path (the TRSM and the next-column update of a step with few
+// tiles).
+template <int TM>
 __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __restrict__ tsn,
                                                   const int* __restrict__ ttr,
                                                   const int* __restrict__ ttc, int mode, int step)
@@ -188,26 +202,26 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
     const int cs = k0 + b;
     double* panel = si.Ls + static_cast<int64_t>(k0) * si.f;   // columns k0..k0+b
     if (mode == 0) {
-      const int R0 = cs + tr * BT, C0 = cs + tc * BT;
+      const int R0 = cs + tr * TM, C0 = cs + tc * BT;
       g.A = panel + R0;
       g.B = panel + C0;
       g.lda = g.ldb = si.f;
       g.C = si.Ls + static_cast<int64_t>(C0) * si.f + R0;
       g.ldc = si.f;
-      g.M = min(BT, si.f - R0);
+      g.M = min(TM, si.f - R0);
       g.N = min(BT, si.nc - C0);
       g.K = b;
       g.alpha = -1.0;
       g.beta = 1.0;
     } else {
-      const int R0 = cs + tr * BT;
+      const int R0 = cs + tr * TM;
       g.A = panel + R0;
       g.lda = si.f;
       g.B = c.dinv + c.dinv_off[tsn[blockIdx.x]] + static_cast<int64_t>(step) * BT * BT;
       g.ldb = BT;
       g.C = panel + R0;
       g.ldc = si.f;
-      g.M = min(BT, si.f - R0);
+      g.M = min(TM, si.f - R0);
       g.N = b;
       g.K = b;
       g.alpha = 1.0;
@@ -215,7 +229,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
     }
   }
   if (g.M <= 0 || g.N <= 0) return;
-  gemm_tile(g, smem);
+  gemm_tile<TM, BT>(g, smem);
 }
 
 // Factor the diagonal block of step `step` (b <= 128) in LDS, write L_kk back
@@ -482,10 +496,13 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
 
 size_t dense_nt_smem() { return 2 * KC * LDA_S * sizeof(double); }
 void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, const int* ttc,
-              int mode, int step, hipStream_t st)
+              int mode, int step, hipStream_t st, bool short_rows)
 {
   if (ntasks <= 0) return;
-  k_dense_nt<<<ntasks, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step);
+  if (short_rows)
+    k_dense_nt<32><<<ntasks, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step);
+  else
+    k_dense_nt<128><<<ntasks, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step);
   HIPCHK(hipGetLastError());
 }
 

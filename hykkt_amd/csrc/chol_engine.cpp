@@ -11,6 +11,8 @@
 // Solves: one forward and one backward sweep over the whole tree.
 #include <algorithm>
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 
 #include "device.hpp"
 
@@ -66,6 +68,8 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   ubuf_.alloc(static_cast<size_t>(std::max<int64_t>(uvo[pl_.ns], 1)));
   dinv_.alloc(static_cast<size_t>(std::max<int64_t>(dvo[pl_.ns], 1)));
   dev::fill(dinv_.p, 0.0, static_cast<int64_t>(dinv_.n), st_);
+  dinvT_.alloc(dinv_.n);
+  dev::fill(dinvT_.p, 0.0, static_cast<int64_t>(dinvT_.n), st_);
   pivot_.alloc(1);
   cd_.n = pl_.n;
   cd_.ns = pl_.ns;
@@ -82,6 +86,7 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   cd_.U = U_.p;
   cd_.pivot = pivot_.p;
   cd_.dinv = dinv_.p;
+  cd_.dinvT = dinvT_.p;
   cd_.dinv_off = dinv_off_.p;
   build_tasks();
   build_solve();
@@ -342,7 +347,16 @@ void CholEngine::build_solve()
   sd_.bpart = bpart_.p;
   sd_.bpart_off = bpart_off_.p;
   sd_.err = flags_.p;
+  sd_.trace = nullptr;
   n_bpart_ = bpo[ns];
+  if (const char* tp = std::getenv("HYKKT_SOLVE_TRACE")) {
+    trace_path_ = tp;
+    const char* at = std::getenv("HYKKT_SOLVE_TRACE_AT");
+    trace_at_ = at ? std::atol(at) : 20;
+    trace_items_.clear();
+    for (auto* v : {&fwd, &fwd_top, &bwd_top, &bwd}) trace_items_.insert(trace_items_.end(), v->begin(), v->end());
+    trace_.alloc(4 * static_cast<size_t>(sd_.n_fwd + sd_.n_fwd_top + sd_.n_bwd_top + sd_.n_bwd) + 4);
+  }
   dev::fill_i(flags_.p, 0, 1, st_);
 }
 
@@ -470,6 +484,24 @@ void CholEngine::solve(double* x)
   dev::fill_sentinel(bpart_.p, n_bpart_, st_);
   sweep(PK_SOLVE_BWD_TOP, 1, false);
   sweep(PK_SOLVE_BWD_FOREST, 0, false);
+  if (trace_at_ >= 0) {
+    // the development timeline: stamps of the solve after the marked one
+    sd_.trace = (n_solve_calls_ == trace_at_) ? trace_.p : nullptr;
+    if (n_solve_calls_ == trace_at_ + 1) {
+      std::vector<long long> h(trace_.n);
+      HIPCHK(hipMemcpyAsync(h.data(), trace_.p, h.size() * sizeof(long long),
+                            hipMemcpyDeviceToHost, st_));
+      HIPCHK(hipStreamSynchronize(st_));
+      if (FILE* f = std::fopen(trace_path_.c_str(), "wb")) {
+        const int cnt[4] = {sd_.n_fwd, sd_.n_fwd_top, sd_.n_bwd_top, sd_.n_bwd};
+        std::fwrite(cnt, sizeof(int), 4, f);
+        std::fwrite(trace_items_.data(), sizeof(int), trace_items_.size(), f);
+        std::fwrite(h.data(), sizeof(long long), h.size() - 4, f);
+        std::fclose(f);
+      }
+    }
+    n_solve_calls_++;
+  }
 }
 
 }  // namespace hykkt

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <utility>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -114,7 +115,7 @@ class CholEngine {
   DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_, sn_parent_;
   DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_, dinv_off_, front_off_;
   DevBuf<int> a_src_;
-  DevBuf<double> L_, U_, ubuf_, dinv_;
+  DevBuf<double> L_, U_, ubuf_, dinv_, dinvT_;
   DevBuf<int> pivot_;
   // task lists
   std::vector<Level> lv_;
@@ -127,6 +128,12 @@ class CholEngine {
   DevBuf<double> bpart_;
   DevBuf<int> fwd_top_items_, bwd_top_items_;
   DevBuf<double> ybuf_;                   // forward-sweep output
+  // development timeline of one solve (HYKKT_SOLVE_TRACE=file, taken at the
+  // HYKKT_SOLVE_TRACE_AT-th solve, default 20)
+  DevBuf<long long> trace_;
+  std::vector<int> trace_items_;
+  std::string trace_path_;
+  long trace_at_ = -1, n_solve_calls_ = 0;
   int64_t n_bpart_ = 0;
   double solve_lower_[2] = {0.0, 0.0};   // stored lower-trapezoid entries: forest, top
   int64_t solve_n_[2] = {0, 0};          // columns: forest, top

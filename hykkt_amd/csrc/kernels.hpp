@@ -119,6 +119,7 @@ struct CholDev {
   int* pivot;               // device int: first failing pivot (min), INT_MAX if none
   double pivot_tol;
   double* dinv;             // inverted 128x128 diagonal blocks (column-major)
+  double* dinvT;            // the same blocks transposed (same offsets)
   const int64_t* dinv_off;  // ns+1, offset of supernode s's first block in dinv
 };
 
@@ -128,7 +129,8 @@ void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task
                 const int* cptr, const int* contrib_child, const int* contrib_col, hipStream_t st);
 // Factor + invert the (<= 128 wide) diagonal block of step `step` of each
 // supernode; max_b = largest block in the launch (selects the LDS footprint).
-// dinv slots must be zero-initialized once (only the b x b lower part is written).
+// dinv and dinvT slots must be zero-initialized once (only the b x b lower,
+// resp. upper, part is written).
 void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st);
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the
@@ -162,6 +164,9 @@ struct SolveDev {
   double* bpart;            // backward chunk partials: chunk u of s at bpart_off[s] + u*nc
   const int64_t* bpart_off;
   int* err;                 // 1: a wait hit its bound (sticky until the host clears it)
+  // Development timeline (null normally): 4 wall-clock stamps per item, items
+  // of the forward forest, forward top, backward top, backward forest in turn.
+  long long* trace;
 };
 // p[i] = the sentinel, i < n
 void fill_sentinel(double* p, int64_t n, hipStream_t st);

@@ -64,16 +64,28 @@ struct SnInfo {
   const double* Ls;
 };
 
+// Wave-uniform copies (scalar registers) of values every lane loaded alike.
+// Buffer descriptors built from them stay scalar; a descriptor the compiler
+// cannot prove uniform turns every buffer load into a serialising loop.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v)
+{
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Supernode s (wave-uniform) of the factor.
 __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 {
   SnInfo i;
-  i.a = c.sn_first[s];
-  i.nc = c.sn_first[s + 1] - i.a;
-  i.nr = c.rows_ptr[s + 1] - c.rows_ptr[s];
+  i.a = uni(c.sn_first[s]);
+  i.nc = uni(c.sn_first[s + 1]) - i.a;
+  i.nr = uni(c.rows_ptr[s + 1]) - uni(c.rows_ptr[s]);
   i.f = i.nc + i.nr;
   i.np = (i.nc + SB - 1) / SB;
   i.nu = (i.nr + SB - 1) / SB;
-  i.Ls = c.L + c.l_off[s];
+  i.Ls = c.L + uni64(c.l_off[s]);
   return i;
 }
 
@@ -112,6 +124,12 @@ __device__ double spin_load(const double* p, int* err)
     v = xload(p);
   }
   return v;
+}
+
+// Stamp k of item `item` in the development timeline (when one is bound).
+__device__ __forceinline__ void stamp(const SolveDev& sd, int item, int k)
+{
+  if (sd.trace && threadIdx.x == 0) sd.trace[4 * static_cast<int64_t>(item) + k] = wall_clock64();
 }
 
 // Fixed-order sum of the NQ slice partials of one row (column).
@@ -209,9 +227,11 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double yv[SB];
   __shared__ double part[NQ][SB];
-  const int s = items[3 * blockIdx.x];
-  const int kind = items[3 * blockIdx.x + 1];
-  const int blk = items[3 * blockIdx.x + 2];
+  const int s = uni(items[3 * blockIdx.x]);
+  const int kind = uni(items[3 * blockIdx.x + 1]);
+  const int blk = uni(items[3 * blockIdx.x + 2]);
+  const int tid = (PF ? sd.n_fwd : 0) + blockIdx.x;   // timeline slot
+  stamp(sd, tid, 0);
   const SnInfo si = sninfo(c, s);
   const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
   const int rend = (kind == 0) ? min(si.nc, r0 + SB) : min(si.f, r0 + SB);
@@ -232,14 +252,16 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   const uint32_t vo = static_cast<uint32_t>(r) * 8u;
   const uint32_t so = static_cast<uint32_t>(q * QW) * f8;
   // row `row` of the inverted diagonal block (upper part stored as zeros)
-  const rsrc_t wv = view(c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB,
+  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(row) * 8u, wso = static_cast<uint32_t>(q * QW) * SB * 8u;
 
+  // Both slices are loaded up front: a load still in flight when the wave
+  // polls for a value delays the poll (loads complete in order).
   double l[PF ? QW : 1], w[PF ? QW : 1];
   if constexpr (PF) {
     if (jmax > 0) slice_load<QW>(l, lview(0), vo, so, f8, lvalid(0), live);
-    if (kind == 0 && jmax == 0) slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
+    if (kind == 0) slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
   }
 
   // right-hand side plus the children's update-vector entries for this row
@@ -251,12 +273,9 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
       v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
   }
+  stamp(sd, tid, 1);
   // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready
   for (int j = 0; j < jmax; j++) {
-    if constexpr (PF) {
-      if (kind == 0 && j == jmax - 1)   // in flight during the last wait
-        slice_load<QW>(w, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
-    }
     const int bj = min(SB, si.nc - j * SB);
     if (threadIdx.x < SB)
       yv[threadIdx.x] = (threadIdx.x < bj) ? spin_load(y + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
@@ -273,6 +292,7 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     if (q == 0) v -= sum_slices<NQ>(part, row);
     __syncthreads();
   }
+  stamp(sd, tid, 2);
   if (kind == 0) {
     // y = Dinv_blk v
     if (q == 0) yv[row] = live ? v : 0.0;
@@ -286,9 +306,159 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   } else {
     if (q == 0 && live) xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v);
   }
+  stamp(sd, tid, 3);
 }
 
 // ---------------------------------------------------------------- backward
+// Column sums of a 128 x 128 block product over 1024 threads: thread
+// (row r = tid % 128, column slice q = tid / 128) holds acc[k], its row's
+// term for column 16 q + k.  Within each wave the 16 values are
+// transpose-reduced over the 64 lanes (each exchange halves the values a
+// lane keeps), then the two row halves of a slice meet in LDS.  Returns with
+// out[c] = sum over the 128 rows for column c (all threads past a barrier).
+// One exchange step: the N values of a lane become N/2 (lanes with bit M set
+// keep the upper half, the others the lower half; each adds its partner's).
+template <int N, int M>
+__device__ __forceinline__ void halve(double (&acc)[16], int lane)
+{
+  const bool hi = (lane & M) != 0;
+#pragma unroll
+  for (int i = 0; i < N / 2; i++) {
+    const double send = hi ? acc[i] : acc[i + N / 2];
+    const double keep = hi ? acc[i + N / 2] : acc[i];
+    acc[i] = keep + __shfl_xor(send, M);
+  }
+}
+
+__device__ __forceinline__ void column_sums(double (&acc)[16], double (*half)[SB], double* out)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  halve<16, 32>(acc, lane);
+  halve<8, 16>(acc, lane);
+  halve<4, 8>(acc, lane);
+  halve<2, 4>(acc, lane);
+  acc[0] += __shfl_xor(acc[0], 2);
+  acc[0] += __shfl_xor(acc[0], 1);
+  // lane l now holds column k(l) = 8 b5 + 4 b4 + 2 b3 + b2 of its slice
+  if ((lane & 3) == 0) {
+    const int k = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                  ((lane >> 2) & 1);
+    half[wave & 1][(wave >> 1) * 16 + k] = acc[0];
+  }
+  __syncthreads();
+  if (threadIdx.x < SB) out[threadIdx.x] = half[0][threadIdx.x] + half[1][threadIdx.x];
+  __syncthreads();
+}
+
+// Top of the tree (1024 threads).  Loads run along rows (lanes = rows of L,
+// coalesced); a thread keeps its row's terms for 16 columns across all the
+// blocks it consumes and the column sums are formed once at the end.
+//   chunk item (u, b): rows [nc + 128u, ...) of s against column block b:
+//     bpart = L(rows, b-block)^T x(rows)
+//   pivot item: columns [128 blk, ...): x = Dinv^T (y - sum of its chunks'
+//     partials - L(later blocks, blk-block)^T x(later blocks))
+__device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& sd, int tslot,
+                                              int s, int kind, int idx, const double* __restrict__ y,
+                                              double* __restrict__ x, double* wl)
+{
+  constexpr int QW = 16;
+  __shared__ double xs[2][SB];
+  __shared__ double half[2][SB];
+  __shared__ double cs[SB];
+  __shared__ double part[8][SB];
+  const SnInfo si = sninfo(c, s);
+  const int r = threadIdx.x & (SB - 1);
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
+  const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
+  double acc[QW];
+#pragma unroll
+  for (int k = 0; k < QW; k++) acc[k] = 0.0;
+  // row r of the block of rows starting at r0 (nrows valid) and columns
+  // starting at c0 (ncols valid): L(r0 + r, c0 + 16 q + k)
+  auto row_slice = [&](double (&l)[QW], int r0, int nrows, int c0, int ncols) {
+    const rsrc_t lv = view(si.Ls + static_cast<int64_t>(c0) * si.f,
+                           static_cast<uint32_t>(ncols) * f8);
+    slice_load<QW>(l, lv, static_cast<uint32_t>(r0 + r) * 8u, static_cast<uint32_t>(q * QW) * f8,
+                   f8, ncols - q * QW, r < nrows);
+  };
+
+  if (kind == 0) {
+    const int u = idx / si.np, b = idx - u * si.np;
+    const int rr0 = u * SB;
+    const int nrows = min(SB, si.nr - rr0);
+    const int bb = min(SB, si.nc - b * SB);
+    double l[QW];
+    row_slice(l, si.nc + rr0, nrows, b * SB, bb);
+    // x of these rows belongs to ancestors: waited for as it is read
+    if (threadIdx.x < SB) {
+      const int* rows = c.rows + c.rows_ptr[s] + rr0;
+      xs[0][threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
+    }
+    __syncthreads();
+    stamp(sd, tslot, 1);
+    const double xr = xs[0][r];
+#pragma unroll
+    for (int k = 0; k < QW; k++) acc[k] = l[k] * xr;
+    column_sums(acc, half, cs);
+    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
+    if (threadIdx.x < bb) xstore(out + threadIdx.x, cs[threadIdx.x]);
+    stamp(sd, tslot, 3);
+    return;
+  }
+
+  const int blk = idx;
+  const int c0 = blk * SB;
+  const int nb = min(SB, si.nc - c0);
+  const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
+  // Dinv^T = the transposed block, read along its columns (coalesced):
+  // x(i) = sum_j dinvT(i, j) t(j), thread (i = r, slice q of j)
+  const rsrc_t wv = view(c.dinvT + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+                         SB * SB * 8u);
+  // The inverse goes to LDS up front (wl[j * 128 + i] = dinvT(i, j)), so
+  // the last link reads no global memory before its result.
+  {
+    double wt[QW];
+    slice_load<QW>(wt, wv, static_cast<uint32_t>(r) * 8u, static_cast<uint32_t>(q * QW) * SB * 8u,
+                   SB * 8u, nb - q * QW, r < nb);
+#pragma unroll
+    for (int k = 0; k < QW; k++) wl[(q * QW + k) * SB + r] = wt[k];
+  }
+  double l[QW];
+  if (jtop > blk) row_slice(l, jtop * SB, min(SB, si.nc - jtop * SB), c0, nb);
+
+  double t = 0.0;
+  if (threadIdx.x < SB && threadIdx.x < nb) {
+    t = y[si.a + c0 + threadIdx.x];
+    const double* bp = sd.bpart + sd.bpart_off[s] + c0 + threadIdx.x;
+    for (int uu = 0; uu < si.nu; uu++) t -= spin_load(bp + static_cast<int64_t>(uu) * si.nc, sd.err);
+  }
+  stamp(sd, tslot, 1);
+  for (int j = jtop; j > blk; j--) {
+    const int bj = min(SB, si.nc - j * SB);
+    double* xb = xs[j & 1];
+    if (threadIdx.x < SB)
+      xb[threadIdx.x] = (threadIdx.x < bj) ? spin_load(x + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
+    __syncthreads();   // xs[j & 1] is written again two blocks on, after the next barrier
+    const double xr = xb[r];
+#pragma unroll
+    for (int k = 0; k < QW; k++) acc[k] += l[k] * xr;
+    if (j - 1 > blk) row_slice(l, (j - 1) * SB, min(SB, si.nc - (j - 1) * SB), c0, nb);
+  }
+  stamp(sd, tslot, 2);
+  column_sums(acc, half, cs);
+  if (threadIdx.x < SB) xs[0][threadIdx.x] = (threadIdx.x < nb) ? t - cs[threadIdx.x] : 0.0;
+  __syncthreads();
+  {
+    double res = 0.0;
+#pragma unroll
+    for (int k = 0; k < QW; k++) res += wl[(q * QW + k) * SB + r] * xs[0][q * QW + k];
+    part[q][r] = res;
+  }
+  __syncthreads();
+  if (threadIdx.x < nb) xstore(x + si.a + c0 + threadIdx.x, sum_slices<8>(part, threadIdx.x));
+  stamp(sd, tslot, 3);
+}
+
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
@@ -298,9 +468,16 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double xs[SB];
   __shared__ double part[NQ][SB];
-  const int s = items[3 * blockIdx.x];
-  const int kind = items[3 * blockIdx.x + 1];
-  const int idx = items[3 * blockIdx.x + 2];
+  const int s = uni(items[3 * blockIdx.x]);
+  const int kind = uni(items[3 * blockIdx.x + 1]);
+  const int idx = uni(items[3 * blockIdx.x + 2]);
+  const int tid = sd.n_fwd + sd.n_fwd_top + (PF ? 0 : sd.n_bwd_top) + blockIdx.x;   // timeline slot
+  stamp(sd, tid, 0);
+  if constexpr (PF) {
+    extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS)
+    solve_bwd_top(c, sd, tid, s, kind, idx, y, x, wl);
+    return;
+  }
   const SnInfo si = sninfo(c, s);
   const int col = threadIdx.x & (SB - 1);
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // whole waves per slice
@@ -327,6 +504,7 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     if (threadIdx.x < SB)
       xs[threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
     __syncthreads();
+    stamp(sd, tid, 1);
     double tp;
     if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
     else tp = slice_dot_stream<QW>(lv, cvo, so, 8u, nrows - q * QW, col < bb, xs + q * QW);
@@ -334,6 +512,7 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     __syncthreads();
     double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
     if (q == 0 && col < bb) xstore(out + col, sum_slices<NQ>(part, col));
+    stamp(sd, tid, 3);
     return;
   }
 
@@ -347,14 +526,15 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   auto lvalid = [&](int j) { return min(SB, si.nc - j * SB) - q * QW; };
   // Dinv_blk^T: x(col) = sum_{row >= col} W(row, col) t(row): column `col` of
   // W (its upper part is stored as zeros)
-  const rsrc_t wv = view(c.dinv + c.dinv_off[s] + static_cast<int64_t>(blk) * SB * SB,
+  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(col) * SB * 8u, wso = static_cast<uint32_t>(q * QW) * 8u;
   const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
+  // both slices up front (a load in flight delays the wave's polls)
   double l[PF ? QW : 1], w[PF ? QW : 1];
   if constexpr (PF) {
     if (jtop > blk) slice_load<QW>(l, lv, cvo, lso(jtop), 8u, lvalid(jtop), live);
-    else slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);
+    slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);
   }
 
   double t = 0.0;
@@ -363,10 +543,8 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     const double* bp = sd.bpart + sd.bpart_off[s] + c0 + col;
     for (int u = 0; u < si.nu; u++) t -= spin_load(bp + static_cast<int64_t>(u) * si.nc, sd.err);
   }
+  stamp(sd, tid, 1);
   for (int j = jtop; j > blk; j--) {
-    if constexpr (PF) {
-      if (j == blk + 1) slice_load<QW>(w, wv, wvo, wso, 8u, nb - q * QW, live);   // last wait
-    }
     const int bj = min(SB, si.nc - j * SB);
     if (threadIdx.x < SB)
       xs[threadIdx.x] = (threadIdx.x < bj) ? spin_load(x + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
@@ -382,6 +560,7 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     if (q == 0) t -= sum_slices<NQ>(part, col);
     __syncthreads();
   }
+  stamp(sd, tid, 2);
   if (q == 0) xs[col] = live ? t : 0.0;
   __syncthreads();
   double res;
@@ -390,6 +569,7 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   part[q][col] = res;
   __syncthreads();
   if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
+  stamp(sd, tid, 3);
 }
 
 }  // namespace
@@ -416,8 +596,16 @@ void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double
 void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
                     hipStream_t st)
 {
-  if (part == 1 && sd.n_bwd_top > 0)
-    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, 0, st>>>(c, sd, sd.bwd_items_top, y, x);
+  if (part == 1 && sd.n_bwd_top > 0) {
+    constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);   // staged inverse block
+    static bool attr = false;
+    if (!attr) {
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(WL)));
+      attr = true;
+    }
+    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top, y, x);
+  }
   if (part == 0 && sd.n_bwd > 0)
     k_solve_bwd<NT_FOREST, false><<<sd.n_bwd, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, y, x);
   HIPCHK(hipGetLastError());

@@ -68,8 +68,6 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   ubuf_.alloc(static_cast<size_t>(std::max<int64_t>(uvo[pl_.ns], 1)));
   dinv_.alloc(static_cast<size_t>(std::max<int64_t>(dvo[pl_.ns], 1)));
   dev::fill(dinv_.p, 0.0, static_cast<int64_t>(dinv_.n), st_);
-  dinvT_.alloc(dinv_.n);
-  dev::fill(dinvT_.p, 0.0, static_cast<int64_t>(dinvT_.n), st_);
   pivot_.alloc(1);
   cd_.n = pl_.n;
   cd_.ns = pl_.ns;
@@ -86,7 +84,6 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   cd_.U = U_.p;
   cd_.pivot = pivot_.p;
   cd_.dinv = dinv_.p;
-  cd_.dinvT = dinvT_.p;
   cd_.dinv_off = dinv_off_.p;
   build_tasks();
   build_solve();

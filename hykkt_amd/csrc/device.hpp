@@ -115,7 +115,7 @@ class CholEngine {
   DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_, sn_parent_;
   DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_, dinv_off_, front_off_;
   DevBuf<int> a_src_;
-  DevBuf<double> L_, U_, ubuf_, dinv_, dinvT_;
+  DevBuf<double> L_, U_, ubuf_, dinv_;
   DevBuf<int> pivot_;
   // task lists
   std::vector<Level> lv_;

@@ -119,7 +119,6 @@ struct CholDev {
   int* pivot;               // device int: first failing pivot (min), INT_MAX if none
   double pivot_tol;
   double* dinv;             // inverted 128x128 diagonal blocks (column-major)
-  double* dinvT;            // the same blocks transposed (same offsets)
   const int64_t* dinv_off;  // ns+1, offset of supernode s's first block in dinv
 };
 
@@ -129,8 +128,7 @@ void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task
                 const int* cptr, const int* contrib_child, const int* contrib_col, hipStream_t st);
 // Factor + invert the (<= 128 wide) diagonal block of step `step` of each
 // supernode; max_b = largest block in the launch (selects the LDS footprint).
-// dinv and dinvT slots must be zero-initialized once (only the b x b lower,
-// resp. upper, part is written).
+// dinv slots must be zero-initialized once (only the b x b lower part is written).
 void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st);
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the

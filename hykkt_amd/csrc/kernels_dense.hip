@@ -489,10 +489,6 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
   for (int col = tid / 64; col < b; col += 4)
     for (int r = (tid & 63) + col; r < b; r += 64) W[static_cast<int64_t>(col) * BT + r] = xinv<BK>(P, r, col);
-  // and transposed (column-major rows of the inverse), for the backward solve
-  double* WT = c.dinvT + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
-  for (int r = tid / 64; r < b; r += 4)
-    for (int col = tid & 63; col <= r; col += 64) WT[static_cast<int64_t>(r) * BT + col] = xinv<BK>(P, r, col);
   if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
 }
 

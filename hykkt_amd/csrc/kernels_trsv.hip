@@ -410,16 +410,16 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
   const int c0 = blk * SB;
   const int nb = min(SB, si.nc - c0);
   const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
-  // Dinv^T = the transposed block, read along its columns (coalesced):
-  // x(i) = sum_j dinvT(i, j) t(j), thread (i = r, slice q of j)
-  const rsrc_t wv = view(c.dinvT + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+  // x = Dinv^T t: x(i) = sum_j Dinv(j, i) t(j), thread (i = r, slice q of j)
+  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
-  // The inverse goes to LDS up front (wl[j * 128 + i] = dinvT(i, j)), so
-  // the last link reads no global memory before its result.
+  // The inverse goes to LDS up front (wl[j * 128 + i] = Dinv(j, i), read
+  // along Dinv's column i), so the last link reads no global memory before
+  // its result.
   {
     double wt[QW];
-    slice_load<QW>(wt, wv, static_cast<uint32_t>(r) * 8u, static_cast<uint32_t>(q * QW) * SB * 8u,
-                   SB * 8u, nb - q * QW, r < nb);
+    slice_load<QW>(wt, wv, static_cast<uint32_t>(r) * SB * 8u, static_cast<uint32_t>(q * QW) * 8u,
+                   8u, nb - q * QW, r < nb);
 #pragma unroll
     for (int k = 0; k < QW; k++) wl[(q * QW + k) * SB + r] = wt[k];
   }

@@ -175,16 +175,22 @@ int hykkt_profile_get(hykkt_solver s, hykkt_kernel_time* out, int capacity, int*
     NEED(s);
     NEED(count);
     HYKKT_REQUIRE(capacity >= 0 && (out || capacity == 0), HYKKT_E_ARG, "bad output buffer");
-    *count = hykkt::PK_COUNT;
-    for (int k = 0; k < hykkt::PK_COUNT && k < capacity; k++) {
+    // the kernels launched since profiling was enabled
+    int n = 0;
+    for (int k = 0; k < hykkt::PK_COUNT; k++) {
       const auto& t = s->impl.prof.total(k);
-      std::memset(out[k].name, 0, sizeof(out[k].name));
-      std::strncpy(out[k].name, hykkt::Profiler::name(k), sizeof(out[k].name) - 1);
-      out[k].launches = t.launches;
-      out[k].ms = t.ms;
-      out[k].flops = t.flops;
-      out[k].bytes = t.bytes;
+      if (t.launches == 0) continue;
+      if (n < capacity) {
+        std::memset(out[n].name, 0, sizeof(out[n].name));
+        std::strncpy(out[n].name, hykkt::Profiler::name(k), sizeof(out[n].name) - 1);
+        out[n].launches = t.launches;
+        out[n].ms = t.ms;
+        out[n].flops = t.flops;
+        out[n].bytes = t.bytes;
+      }
+      n++;
     }
+    *count = n;
     return HYKKT_OK;
   });
 }
@@ -518,6 +524,39 @@ int hykkt_chol_analyze_host(int n, const int* rp, const int* ci, long long* nnz_
     if (ns) *ns = p.ns;
     if (nlev) *nlev = static_cast<int>(p.level_ptr.size()) - 1;
     if (max_front) *max_front = p.max_front;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_flow_selfcheck(int n, const int* rp, const int* ci, long long* ntasks, int* nlevels)
+{
+  return guard([&] {
+    NEED(rp);
+    hykkt::Pattern a = make_pattern(n, n, rp, ci);
+    hykkt::CholPlan p = hykkt::chol_analyze(a, hykkt::CholOptions{});
+    std::vector<int64_t> cnt_off(p.ns + 1, 0);
+    int64_t ncnt = 0;
+    long long tot = 0;
+    int nl = 0;
+    for (size_t l = 0; l + 1 < p.level_ptr.size(); l++) {
+      const int b0 = p.level_ptr[l], b1 = p.level_ptr[l + 1];
+      int nsteps = 0;
+      for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, (p.nc(p.level_sn[q]) + 127) / 128);
+      if (nsteps < 2) continue;
+      std::vector<int> flow;
+      double fl = 0.0, by = 0.0;
+      const int64_t lo = ncnt;
+      hykkt::flow_tasks(p, b0, b1, cnt_off, ncnt, flow, fl, by);
+      const int ntk = static_cast<int>(flow.size() / 4);
+      const int rc = hykkt::flow_check(p, b0, b1, cnt_off, lo, ncnt - lo, flow.data(), ntk);
+      HYKKT_REQUIRE(rc == 0, HYKKT_E_STATE,
+                    "dataflow task list of level " + std::to_string(l) + " fails its replay (" +
+                        std::to_string(rc) + ")");
+      tot += ntk;
+      nl++;
+    }
+    if (ntasks) *ntasks = tot;
+    if (nlevels) *nlevels = nl;
     return HYKKT_OK;
   });
 }

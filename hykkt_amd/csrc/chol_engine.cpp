@@ -10,6 +10,7 @@
 //   extend-add of the children's update matrices into U.
 // Solves: one forward and one backward sweep over the whole tree.
 #include <algorithm>
+#include <queue>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -34,6 +35,12 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming));
   pl_ = chol_analyze(a, opt);
+  {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev));
+    n_cu_ = std::max(n_cu_, 1);
+  }
   auto upi = [](DevBuf<int>& b, const std::vector<int>& v) {
     if (v.empty()) b.upload(std::vector<int>{0});
     else b.upload(v);
@@ -98,6 +105,14 @@ CholEngine::~CholEngine()
 
 void CholEngine::build_tasks()
 {
+  // The dataflow path is off unless HYKKT_FACTOR_FLOW=1: its first run on the
+  // device did not finish and the cause is not established.
+  const char* fe = std::getenv("HYKKT_FACTOR_FLOW");
+  const bool use_flow = fe && fe[0] == '1';
+  if (const char* sp = std::getenv("HYKKT_FLOW_SPIN")) flow_spin_ = std::max(1, std::atoi(sp));
+  std::vector<int> flow;
+  std::vector<int64_t> cnt_off(pl_.ns + 1, 0);
+  int64_t ncnt = 0;
   const int nlev = static_cast<int>(pl_.level_ptr.size()) - 1;
   lv_.assign(nlev, Level{});
   std::vector<int> eap, eacol, eacptr{0}, eacc, eaccol;
@@ -145,6 +160,11 @@ void CholEngine::build_tasks()
     }
     int nsteps = 0;
     for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, cdiv(pl_.nc(pl_.level_sn[q]), NB));
+    if (use_flow && nsteps >= 2) {
+      L.use_flow = true;
+      build_flow(L, b0, b1, flow, cnt_off, ncnt);
+      nsteps = 0;   // no per-step launches
+    }
     L.potrf.resize(nsteps);
     L.trsm.resize(nsteps);
     L.upd_next.resize(nsteps);
@@ -257,6 +277,25 @@ void CholEngine::build_tasks()
   up(t_syrk_sn_, syrk_sn);
   up(t_syrk_tr_, syrk_tr);
   up(t_syrk_tc_, syrk_tc);
+  up(t_flow_, flow);
+  flow_cnt_off_.upload(cnt_off);
+  flow_cnt_.alloc(static_cast<size_t>(std::max<int64_t>(ncnt, 1)));
+  flow_next_.alloc(1);
+  flow_err_.alloc(1);
+  dev::fill_i(flow_err_.p, 0, 1, st_);
+}
+
+void CholEngine::build_flow(Level& L, int b0, int b1, std::vector<int>& flow,
+                            std::vector<int64_t>& cnt_off, int64_t& ncnt)
+{
+  L.flow.off = static_cast<int>(flow.size() / 4);
+  L.cnt_lo = ncnt;
+  double flops = 0.0, bytes = 0.0;
+  flow_tasks(pl_, b0, b1, cnt_off, ncnt, flow, flops, bytes);
+  L.cnt_n = ncnt - L.cnt_lo;
+  L.flow.cnt = static_cast<int>(flow.size() / 4) - L.flow.off;
+  L.flow.flops = flops;
+  L.flow.bytes = bytes;
 }
 
 void CholEngine::build_solve()
@@ -396,6 +435,15 @@ void CholEngine::factor(double pivot_tol)
     // step j-1 also writes, so it waits for that (ev_rest_); the rest of
     // step j waits for step j's TRSM (ev_main_).  Both events are re-recorded
     // every step after their waits were enqueued.
+    if (L.use_flow) {
+      dev::fill_i(flow_cnt_.p + L.cnt_lo, 0, L.cnt_n, st_);
+      dev::fill_i(flow_next_.p, 0, 1, st_);
+      timed(PK_FLOW, L.flow, [&] {
+        dev::FlowDev fd{t_flow_.p + 4 * static_cast<int64_t>(L.flow.off), L.flow.cnt,
+                        flow_next_.p, flow_cnt_.p, flow_cnt_off_.p, flow_err_.p, flow_spin_};
+        dev::factor_flow(cd_, fd, n_cu_, st_);
+      });
+    }
     bool rest_pending = false;
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
@@ -441,9 +489,27 @@ void CholEngine::factor(double pivot_tol)
 
 int CholEngine::pivot()
 {
-  int h = INT_MAX;
+  int h = INT_MAX, fe = 0;
   HIPCHK(hipMemcpyAsync(&h, pivot_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(&fe, flow_err_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
+  if (fe) {
+    // which tiles were short: the first few counters below their final value
+    std::vector<int> cnt(flow_cnt_.n);
+    int nx = 0;
+    HIPCHK(hipMemcpy(cnt.data(), flow_cnt_.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&nx, flow_next_.p, sizeof(int), hipMemcpyDeviceToHost));
+    std::string msg = "factorization: a dataflow wait exceeded its bound (tasks taken " +
+                      std::to_string(nx) + ")";
+    for (const Level& L : lv_) {
+      if (!L.use_flow) continue;
+      msg += "; level counters [" + std::to_string(L.cnt_lo) + ", +" + std::to_string(L.cnt_n) + "]:";
+      for (int64_t i = L.cnt_lo; i < L.cnt_lo + L.cnt_n && i < L.cnt_lo + 24; i++)
+        msg += " " + std::to_string(cnt[i]);
+    }
+    dev::fill_i(flow_err_.p, 0, 1, st_);
+    throw Error(HYKKT_E_HIP, msg);
+  }
   return h == INT_MAX ? -1 : h;
 }
 

@@ -11,6 +11,7 @@
 // consecutive rows of one C column (coalesced column-major stores).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "kernels.hpp"
@@ -25,6 +26,22 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int BT = 128;        // tile edge (rows and columns of C)
 constexpr int KC = 16;         // K chunk staged in LDS
 constexpr int LDA_S = BT + 2;  // LDS row length (doubles)
+
+// Global loads and stores of factor data.  COH: device-scope relaxed atomics
+// (coherent across the XCDs' L2 caches), for data another workgroup of the
+// same launch produced or will consume.
+template <bool COH>
+__device__ __forceinline__ double ldg(const double* p)
+{
+  if constexpr (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void stg(double* p, double v)
+{
+  if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
 
 struct GemmArgs {
   const double* A;
@@ -47,7 +64,7 @@ struct GemmArgs {
 // once) and are stored once at the end; alpha is applied exactly to the
 // staged A values.  K is staged through LDS in chunks of KC, the next chunk's
 // global loads overlapping the current chunk's MFMAs.
-template <int TM, int TN>
+template <int TM, int TN, bool COH = false>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 {
   constexpr int WN = TM == TN ? 2 : 4, WM = 4 / WN;
@@ -70,7 +87,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int c = wc + j * 16 + 4 * v + (lane >> 4);
-          acc[i][j][v] = (r < g.M && c < g.N) ? g.C[c * g.ldc + r] : 0.0;
+          acc[i][j][v] = (r < g.M && c < g.N) ? ldg<COH>(g.C + c * g.ldc + r) : 0.0;
         }
     }
   } else {
@@ -88,12 +105,12 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 #pragma unroll
     for (int q = 0; q < EA; q++) {
       const int r = sa + q;
-      ra[q] = (kk < g.K && r < g.M) ? g.A[kk * g.lda + r] : 0.0;
+      ra[q] = (kk < g.K && r < g.M) ? ldg<COH>(g.A + kk * g.lda + r) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < EB; q++) {
       const int r = sb + q;
-      rb[q] = (kk < g.K && r < g.N) ? g.B[kk * g.ldb + r] : 0.0;
+      rb[q] = (kk < g.K && r < g.N) ? ldg<COH>(g.B + kk * g.ldb + r) : 0.0;
     }
   };
   gload(0);
@@ -130,7 +147,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
         for (int v = 0; v < 4; v++) {
           const int c = wc + j * 16 + 4 * v + (lane >> 4);
           const int64_t gr = g.pr0 + r, gc = g.pc0 + c;
-          if (r < g.M && c < g.N && gr >= gc) g.C[packed_lower(gr, gc, g.pk)] = acc[i][j][v];
+          if (r < g.M && c < g.N && gr >= gc) stg<COH>(g.C + packed_lower(gr, gc, g.pk), acc[i][j][v]);
         }
     }
     return;
@@ -143,7 +160,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int c = wc + j * 16 + 4 * v + (lane >> 4);
-        if (r < g.M && c < g.N) g.C[c * g.ldc + r] = acc[i][j][v];
+        if (r < g.M && c < g.N) stg<COH>(g.C + c * g.ldc + r, acc[i][j][v]);
       }
   }
 }
@@ -341,15 +358,12 @@ __device__ __forceinline__ void inverse_level(double (*P)[BK + 1], int wave, int
   __syncthreads();
 }
 
-// BK: padded block edge (32, 64 or 128) chosen per launch from the largest
-// diagonal block in it, so small supernodes need little LDS.
-template <int BK>
-__global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
-                                                   int step)
+// Diagonal block `step` of supernode s, by one workgroup of 256 threads with
+// smem >= potrf_smem<BK>().  BK: padded block edge (32, 64 or 128) >= b.
+template <int BK, bool COH>
+__device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
 {
   constexpr int NPN = BK / PW;   // panels
-  extern __shared__ double smem[];
-  const int s = sns[blockIdx.x];
   const SnInfo si = sninfo(c, s);
   const int k0 = step * BT;
   if (k0 >= si.nc) return;
@@ -367,7 +381,7 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
 #pragma unroll
     for (int u = 0; u < PER; u++) {
       const int e = e0 + u * 256, r = e % BK, col = e / BK;
-      v[u] = (e < NE && r >= col && r < b) ? D[r + static_cast<int64_t>(col) * si.f]
+      v[u] = (e < NE && r >= col && r < b) ? ldg<COH>(D + r + static_cast<int64_t>(col) * si.f)
                                            : (r == col ? 1.0 : 0.0);
     }
 #pragma unroll
@@ -483,13 +497,146 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   if constexpr (BK >= 128) inverse_level<BK, 64>(P, wave, lane);
 
   for (int col = tid / 64; col < b; col += 4)
-    for (int r = (tid & 63) + col; r < b; r += 64) D[r + static_cast<int64_t>(col) * si.f] = P[r][col];
+    for (int r = (tid & 63) + col; r < b; r += 64) stg<COH>(D + r + static_cast<int64_t>(col) * si.f, P[r][col]);
   // inverse into the 128 x 128 dinv slot (column-major); entries outside the
   // b x b lower triangle stay at the zeros written when the slot was created
   double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
   for (int col = tid / 64; col < b; col += 4)
-    for (int r = (tid & 63) + col; r < b; r += 64) W[static_cast<int64_t>(col) * BT + r] = xinv<BK>(P, r, col);
+    for (int r = (tid & 63) + col; r < b; r += 64) stg<COH>(W + static_cast<int64_t>(col) * BT + r, xinv<BK>(P, r, col));
   if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
+}
+
+template <int BK>
+__global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
+                                                   int step)
+{
+  extern __shared__ double smem[];
+  const int s = sns[blockIdx.x];
+  if (step * BT >= c.sn_first[s + 1] - c.sn_first[s]) return;
+  potrf_block<BK, false>(c, s, step, smem);
+}
+
+// ---------------------------------------------------------------- dataflow
+// Panel factorization of one tree level as tasks over 128 x 128 tiles of each
+// supernode's panel, executed by persistent workgroups in list order (the
+// list is a topological order, so a task only waits for tasks taken before
+// it and the launch cannot deadlock).  Tasks (4 ints): type | short << 2 |
+// quarter << 4, supernode, step k, R | C << 16.
+//   D(k):       diagonal block k (factor + inverse)
+//   T(k, R):    rows of row block R below block k times Dinv_k^T
+//   U(k, R, C): tile (R, C) -= L(R, k) L(C, k)^T
+// Each tile (R, C) of a supernode has a counter of the operations applied to
+// it, in quarter units (a short task covers 32 of the 128 rows and adds 1, a
+// full task adds 4): updates from steps 0..C-1, then D(C) or T(C, R).  A
+// task's inputs are complete when the counters reach their marks; its
+// results are stored coherently and complete (s_waitcnt) before its counter
+// moves.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Wait until *p >= want.  Gives up (sets *err) after `spin` polls or once
+// another wait gave up, so the launch always drains.
+__device__ void flow_wait(const int* p, int want, int* err, int spin)
+{
+  int it = 0;
+  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+    __builtin_amdgcn_s_sleep(4);
+    ++it;
+    if (it > spin ||
+        ((it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+}
+
+// Task bodies, kept out of line so each gets the register file to itself.
+__device__ __noinline__ void flow_diag(const CholDev& c, int s, int k, int b, double* smem)
+{
+  if (b > 64) potrf_block<128, true>(c, s, k, smem);
+  else if (b > 32) potrf_block<64, true>(c, s, k, smem);
+  else potrf_block<32, true>(c, s, k, smem);
+}
+__device__ __noinline__ void flow_gemm(const GemmArgs& g, bool shrt, double* smem)
+{
+  if (shrt) gemm_tile<32, BT, true>(g, smem);
+  else gemm_tile<BT, BT, true>(g, smem);
+}
+
+__global__ void __launch_bounds__(256) k_factor_flow(CholDev c, FlowDev fd)
+{
+  extern __shared__ double smem[];
+  __shared__ int tsk;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) tsk = atomicAdd(fd.next, 1);
+    __syncthreads();
+    const int t = tsk;
+    __syncthreads();   // every thread has read tsk before it is taken again
+    if (t >= fd.ntasks) return;
+    const int w0 = uni(fd.tasks[4 * t]), s = uni(fd.tasks[4 * t + 1]);
+    const int k = uni(fd.tasks[4 * t + 2]), rc = uni(fd.tasks[4 * t + 3]);
+    const int type = w0 & 3, shrt = (w0 >> 2) & 1, qr = (w0 >> 4) & 3;
+    const int R = rc & 0xffff, C = rc >> 16;
+    const SnInfo si = sninfo(c, s);
+    const int ncb = (si.nc + BT - 1) / BT;
+    int* cnt = fd.cnt + fd.cnt_off[s];
+    auto tile = [&](int r, int cc) { return cnt + r * ncb + cc; };
+    if (tid == 0) {
+      if (type == 0) {
+        flow_wait(tile(k, k), 4 * k, fd.err, fd.spin);
+      } else if (type == 1) {
+        flow_wait(tile(k, k), 4 * k + 4, fd.err, fd.spin);
+        if (R != k) flow_wait(tile(R, k), 4 * k, fd.err, fd.spin);
+      } else {
+        flow_wait(tile(R, k), 4 * k + 4, fd.err, fd.spin);
+        flow_wait(tile(C, k), 4 * k + 4, fd.err, fd.spin);
+        flow_wait(tile(R, C), 4 * k, fd.err, fd.spin);
+      }
+    }
+    __syncthreads();
+    const int k0 = k * BT, b = min(BT, si.nc - k0);
+    if (type == 0) {
+      flow_diag(c, s, k, b, smem);
+    } else {
+      double* panel = si.Ls + static_cast<int64_t>(k0) * si.f;   // columns of block k
+      const int rlo = R * BT + (shrt ? 32 * qr : 0);
+      const int rhi = min(R * BT + (shrt ? 32 * qr + 32 : BT), si.f);
+      GemmArgs g;
+      if (type == 1) {
+        const int R0 = max(rlo, k0 + b);
+        g.A = panel + R0;
+        g.lda = si.f;
+        g.B = c.dinv + c.dinv_off[s] + static_cast<int64_t>(k) * BT * BT;
+        g.ldb = BT;
+        g.C = panel + R0;
+        g.ldc = si.f;
+        g.M = rhi - R0;
+        g.N = b;
+        g.K = b;
+        g.alpha = 1.0;
+        g.beta = 0.0;
+      } else {
+        const int C0 = C * BT;
+        g.A = panel + rlo;
+        g.B = panel + C0;
+        g.lda = g.ldb = si.f;
+        g.C = si.Ls + static_cast<int64_t>(C0) * si.f + rlo;
+        g.ldc = si.f;
+        g.M = rhi - rlo;
+        g.N = min(BT, si.nc - C0);
+        g.K = b;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+      }
+      if (g.M > 0 && g.N > 0) flow_gemm(g, shrt != 0, smem);
+    }
+    __builtin_amdgcn_s_waitcnt(0);   // this thread's results have landed
+    __syncthreads();
+    if (tid == 0) {
+      int* ct = (type == 0) ? tile(k, k) : (type == 1 ? tile(R, k) : tile(R, C));
+      __hip_atomic_fetch_add(ct, shrt ? 1 : 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 }  // namespace
@@ -524,6 +671,22 @@ static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hi
     attr = true;
   }
   k_potrf_inv<BK><<<nsn, 256, potrf_smem<BK>(), st>>>(c, sns, step);
+  HIPCHK(hipGetLastError());
+}
+
+size_t factor_flow_smem() { return std::max(potrf_smem<128>(), dense_nt_smem()); }
+
+void factor_flow(const CholDev& c, const FlowDev& f, int grid, hipStream_t st)
+{
+  if (f.ntasks <= 0) return;
+  static bool attr = false;
+  if (!attr) {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_flow),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(factor_flow_smem())));
+    attr = true;
+  }
+  k_factor_flow<<<std::min(grid, f.ntasks), 256, factor_flow_smem(), st>>>(c, f);
   HIPCHK(hipGetLastError());
 }
 

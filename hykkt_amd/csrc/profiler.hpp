@@ -20,6 +20,7 @@ enum ProfKind {
   PK_SOLVE_FWD_TOP,
   PK_SOLVE_BWD_TOP,
   PK_SOLVE_BWD_FOREST,
+  PK_FLOW,
   PK_COUNT
 };
 

@@ -255,13 +255,12 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
 // identity: L(r, c) (r >= c) at P[r][c], the inverse X(i, j) (i > j) at
 // P[j][i] and X(j, j) at P[j][BK]; and the reciprocals of L's diagonal.
 //
-// Cholesky, blocked by 16-column panels: wave 0 factors the panel's 16x16
-// diagonal block in registers (lane r holds row r; column k reaches every
-// lane through v_readlane; 1/sqrt of the pivot is v_rsq_f64 refined by two
-// Newton steps, so the chain carries no division or square root); one thread
-// per row below solves that row against the block (multiplying by the
-// reciprocals); the trailing lower part takes the rank-16 update as FP64 MFMA
-// 16x16x4 tiles spread over the four waves.  Inverse: the 16x16 diagonal
+// Cholesky, blocked by 16-column panels: wave 0 factors the whole panel in
+// registers (two rows per lane; the pivot row's entries reach every lane
+// through v_readlane; 1/sqrt of the pivot is v_rsq_f64 refined by two Newton
+// steps, so the chain carries no division or square root); the trailing
+// lower part takes the rank-16 update as FP64 MFMA 16x16x4 tiles spread over
+// the four waves.  Inverse: the 16x16 diagonal
 // blocks are inverted together (16 lanes per block, lane j computes column
 // j), then blocks of 16, 32 and 64 are doubled with
 //   inv([L11 0; L21 L22]) = [X11 0; -X22 (L21 X11)  X22],
@@ -302,66 +301,106 @@ __device__ __forceinline__ double xinv(const double (*P)[BK + 1], int i, int j)
 }
 
 // One doubling level: for every pair of inverted diagonal blocks of size H,
-// X21 = -X22 (L21 X11).  Tile t of NT = pairs * (H/16)^2 goes to wave t % 4.
+// X21 = -X22 (L21 X11), in 16 x 16 tiles (HB = H/16 per side).  The pair's
+// HB waves take HB tiles each: in T = L21 X11 wave wp takes tile columns
+// cb = u (u < HB) in rows (wp + u) % HB; in -X22 T, tile rows rb = u in
+// columns (wp + u) % HB.  As X11 and X22 are lower triangular, a tile takes
+// the k-blocks kk >= cb (resp. kk <= rb) only, so every wave does the same
+// HB (HB + 1) / 2 block products.  All of a wave's operands are read from
+// LDS before its MFMA chains run, interleaved.
 template <int BK, int H>
 __device__ __forceinline__ void inverse_level(double (*P)[BK + 1], int wave, int lane)
 {
-  constexpr int HB = H / 16, TPP = HB * HB, NT = (BK / (2 * H)) * TPP, PERW = (NT + 3) / 4;
-  int r0[PERW], c0[PERW], b1[PERW];
-#pragma unroll
-  for (int u = 0; u < PERW; u++) {
-    const int t = wave + 4 * u, q = t / TPP, tt = t - q * TPP;
-    b1[u] = 2 * q * H;
-    r0[u] = b1[u] + H + 16 * (tt / HB);
-    c0[u] = b1[u] + 16 * (tt % HB);
-  }
+  constexpr int HB = H / 16, NPR = BK / (2 * H), WPP = 4 / NPR;   // waves per pair
+  const int pair = wave / WPP, wp = wave % WPP;
+  const bool act = wp < HB;
+  const int b1 = 2 * H * pair;
   const int i = lane & 15, kq = lane >> 4;
+  d4 acc[HB];
+  double fa[HB][HB][4], fb[HB][HB][4];
   // T = L21 X11, stored where X21 goes (T(r, c) at P[c][r])
-  d4 acc[PERW];
+  if (act) {
 #pragma unroll
-  for (int u = 0; u < PERW; u++) {
-    acc[u] = d4{0.0, 0.0, 0.0, 0.0};
-    if (wave + 4 * u < NT) {
+    for (int u = 0; u < HB; u++) {
+      const int r0 = b1 + H + 16 * ((wp + u) % HB), c0 = b1 + 16 * u;
 #pragma unroll
-      for (int k0 = 0; k0 < H; k0 += 4) {
-        const int k = b1[u] + k0 + kq;
-        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[r0[u] + i][k], xinv<BK>(P, k, c0[u] + i),
-                                                      acc[u], 0, 0, 0);
-      }
+      for (int kk = u; kk < HB; kk++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const int k = b1 + 16 * kk + 4 * m + kq;
+          fa[u][kk][m] = P[r0 + i][k];
+          fb[u][kk][m] = (kk == u) ? xinv<BK>(P, k, c0 + i) : P[c0 + i][k];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < HB; u++) acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < HB; kk++)
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int u = 0; u <= kk; u++)
+          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][kk][m], fb[u][kk][m], acc[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < HB; u++) {
+      const int r0 = b1 + H + 16 * ((wp + u) % HB), c0 = b1 + 16 * u;
+#pragma unroll
+      for (int v = 0; v < 4; v++) P[c0 + i][r0 + 4 * v + kq] = acc[u][v];
     }
   }
-#pragma unroll
-  for (int u = 0; u < PERW; u++)
-    if (wave + 4 * u < NT)
-#pragma unroll
-      for (int v = 0; v < 4; v++) P[c0[u] + i][r0[u] + 4 * v + kq] = acc[u][v];
   __syncthreads();
   // X21 = -X22 T; every read of T before the in-place writes
+  if (act) {
 #pragma unroll
-  for (int u = 0; u < PERW; u++) {
-    acc[u] = d4{0.0, 0.0, 0.0, 0.0};
-    if (wave + 4 * u < NT) {
+    for (int u = 0; u < HB; u++) {
+      const int r0 = b1 + H + 16 * u, c0 = b1 + 16 * ((wp + u) % HB);
 #pragma unroll
-      for (int k0 = 0; k0 < H; k0 += 4) {
-        const int k = b1[u] + H + k0 + kq;
-        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xinv<BK>(P, r0[u] + i, k), P[c0[u] + i][k],
-                                                      acc[u], 0, 0, 0);
-      }
+      for (int kk = 0; kk <= u; kk++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const int k = b1 + H + 16 * kk + 4 * m + kq;
+          fa[u][kk][m] = (kk == u) ? xinv<BK>(P, r0 + i, k) : P[k][r0 + i];
+          fb[u][kk][m] = P[c0 + i][k];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < HB; u++) acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < HB; kk++)
+#pragma unroll
+      for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int u = kk; u < HB; u++)
+          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][kk][m], fb[u][kk][m], acc[u], 0, 0, 0);
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < HB; u++) {
+      const int r0 = b1 + H + 16 * u, c0 = b1 + 16 * ((wp + u) % HB);
+#pragma unroll
+      for (int v = 0; v < 4; v++) P[c0 + i][r0 + 4 * v + kq] = -acc[u][v];
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int u = 0; u < PERW; u++)
-    if (wave + 4 * u < NT)
-#pragma unroll
-      for (int v = 0; v < 4; v++) P[c0[u] + i][r0[u] + 4 * v + kq] = -acc[u][v];
-  __syncthreads();
+}
+
+// Entry e < BK (BK + 1) / 2 of the lower triangle of a BK x BK block (BK
+// even), columns folded in pairs c, BK-1-c (together BK + 1 entries), so
+// consecutive e run down one column.
+template <int BK>
+__device__ __forceinline__ void lower_entry(int e, int& r, int& col)
+{
+  const int pr = e / (BK + 1), o = e - pr * (BK + 1);
+  const bool first = o < BK - pr;
+  col = first ? pr : BK - 1 - pr;
+  r = first ? pr + o : col + o - (BK - pr);
 }
 
 // Diagonal block `step` of supernode s, by one workgroup of 256 threads with
 // smem >= potrf_smem<BK>().  BK: padded block edge (32, 64 or 128) >= b.
 template <int BK, bool COH>
-__device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
+__device__ __forceinline__ void potrf_block(const CholDev& c, int s, int step, double* smem)
 {
   constexpr int NPN = BK / PW;   // panels
   const SnInfo si = sninfo(c, s);
@@ -371,23 +410,29 @@ __device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
   double (*P)[BK + 1] = reinterpret_cast<double (*)[BK + 1]>(smem);
   double* rd = smem + BK * (BK + 1);   // 1 / L(r, r)
   __shared__ int bad;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // scalar tile bookkeeping
   double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
-  // load the block, up to 32 elements per thread in flight: element
-  // e = col*BK + r, lower part only, padded with the identity
-  constexpr int NE = BK * BK, PER = NE / 256 < 32 ? NE / 256 : 32, STEP = 256 * PER;
-  for (int e0 = tid; e0 < NE; e0 += STEP) {
+  // load the lower triangle (padded with the identity past b), all of a
+  // thread's entries in flight at once.  Every lane loads (entries past b or
+  // past the triangle read D[0] and are replaced), so the loads are not split
+  // by branches that would wait on each other.
+  constexpr int NL = BK * (BK + 1) / 2, PER = (NL + 255) / 256;
+  {
     double v[PER];
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-      const int e = e0 + u * 256, r = e % BK, col = e / BK;
-      v[u] = (e < NE && r >= col && r < b) ? ldg<COH>(D + r + static_cast<int64_t>(col) * si.f)
-                                           : (r == col ? 1.0 : 0.0);
+      int r, col;
+      lower_entry<BK>(min(tid + u * 256, NL - 1), r, col);
+      const bool in = r < b;
+      const double t = ldg<COH>(D + (in ? r + static_cast<int64_t>(col) * si.f : 0));
+      v[u] = in ? t : (r == col ? 1.0 : 0.0);
     }
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-      const int e = e0 + u * 256, r = e % BK, col = e / BK;
-      if (e < NE && r >= col) P[r][col] = v[u];
+      int r, col;
+      lower_entry<BK>(min(tid + u * 256, NL - 1), r, col);
+      if (tid + u * 256 < NL) P[r][col] = v[u];
     }
   }
   if (tid == 0) bad = INT_MAX;
@@ -395,22 +440,34 @@ __device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
   const double tol2 = c.pivot_tol * c.pivot_tol;
 
   // ---- blocked Cholesky ----
+  // Each wave factors the panel's 16 x 16 diagonal block in lanes 0..15 (lane
+  // l holds row kb + l; every wave the same) and 48 of the rows below in
+  // lanes 16..63, so the whole panel is factored in one pass: at step k the
+  // pivot row's entries reach every lane of the wave through v_readlane
+  // (read before the pivot's 1/sqrt is known, scaled after), column k is
+  // scaled and the later columns take its rank-1 update.  Entries above the
+  // diagonal of the 16 x 16 block are carried along, never stored, so no
+  // lane-dependent branches.
+  {
+    const bool dg = lane < PW;
+    const int prow_off = dg ? lane : PW + 48 * wave + (lane - PW);   // row - kb
 #pragma unroll 1
-  for (int p = 0; p < NPN; p++) {
-    const int kb = p * PW;
-    if (wave == 0) {
-      // lane r holds row r of the diagonal block (lanes >= 16 repeat rows
-      // 0..15 and store nothing); entries above the diagonal are carried
-      // along, never stored, so no lane-dependent branches.
-      const int r = lane & (PW - 1);
+    for (int p = 0; p < NPN; p++) {
+      const int kb = p * PW;
+      const int r = kb + prow_off;
+      const bool mine = r < BK && (dg ? wave == 0 : true);   // this lane stores row r
+      const int rr = min(r, BK - 1);
       double v[PW];
 #pragma unroll
-      for (int q = 0; q < PW; q++) v[q] = P[kb + r][kb + q];
+      for (int q = 0; q < PW; q++) v[q] = P[rr][kb + q];
       int badk = -1;
-      double myinv = 1.0;   // 1 / L(r, r), taken at step k == r (no dynamic register index)
+      double myinv = 1.0;   // 1 / L(kb + lane, kb + lane), taken at step k == lane
 #pragma unroll
       for (int k = 0; k < PW; k++) {
         const double dkk = readlane_f64(v[k], k);
+        double lu[PW];   // unscaled entries (kb+q, kb+k) of rows q > k
+#pragma unroll
+        for (int q = k + 1; q < PW; q++) lu[q] = readlane_f64(v[k], q);
         const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
         badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
         const double xk = ok ? dkk : 1.0;   // failure reported, arithmetic kept finite
@@ -418,57 +475,81 @@ __device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
         const double hx = 0.5 * xk;
         y = y * fma(-hx, y * y, 1.5);
         y = y * fma(-hx, y * y, 1.5);
-        v[k] = (r == k) ? xk * y : v[k] * y;
-        myinv = (r == k) ? y : myinv;
-        const double vk = v[k];
+        // L(r, kb+k) = v_k y; the update L(r, kb+k) L(kb+q, kb+k) is v_k y^2 lu_q
+        const double wk = v[k] * (y * y);
+        v[k] = (lane == k) ? xk * y : v[k] * y;
+        myinv = (lane == k) ? y : myinv;
 #pragma unroll
-        for (int q = k + 1; q < PW; q++) v[q] -= vk * readlane_f64(vk, q);
+        for (int q = k + 1; q < PW; q++) v[q] -= wk * lu[q];
       }
-      if (lane == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
-      if (lane < PW) {
+      if (tid == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
+      if (mine) {
 #pragma unroll
         for (int q = 0; q < PW; q++)
-          if (q <= r) P[kb + r][kb + q] = v[q];
-        rd[kb + r] = myinv;
+          if (!dg || q <= lane) P[r][kb + q] = v[q];
       }
-    }
-    __syncthreads();
+      if (wave == 0 && dg) rd[kb + lane] = myinv;
+      __syncthreads();
     const int rbeg = kb + PW;
     if (rbeg < BK) {
-      // rows below: thread t solves row rbeg + t against the diagonal block
-      if (tid < BK - rbeg) {
-        const int r = rbeg + tid;
-        double x[PW];
-#pragma unroll
-        for (int j = 0; j < PW; j++) {
-          double t0 = P[r][kb + j], t1 = 0.0;
-#pragma unroll
-          for (int k = 0; k < j; k++) {
-            if (k & 1) t1 -= x[k] * P[kb + j][kb + k];
-            else t0 -= x[k] * P[kb + j][kb + k];
-          }
-          x[j] = (t0 + t1) * rd[kb + j];
-        }
-#pragma unroll
-        for (int j = 0; j < PW; j++) P[r][kb + j] = x[j];
-      }
-      __syncthreads();
       // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k] over the panel,
       // r >= cc >= rbeg, in 16x16 tiles (lower tiles only; the upper half of
       // a diagonal tile is scratch until the inverse is written)
-      const int nrt = (BK - rbeg) / PW;
-      for (int e = wave; e < nrt * (nrt + 1) / 2; e += 4) {
-        int ti = 0;
-        while ((ti + 1) * (ti + 2) / 2 <= e) ti++;
-        const int tj = e - ti * (ti + 1) / 2;
-        const int r0 = rbeg + PW * ti, c0 = rbeg + PW * tj;
-        const d4 a = mfma16<PW>(lane, [&](int i, int k) { return P[r0 + i][kb + k]; },
-                                [&](int k, int j) { return P[c0 + j][kb + k]; });
+      // Tile e goes to wave e % 4; a wave takes two tiles at a time (all
+      // operand reads first, the two MFMA chains interleaved).
+      const int nrt = (BK - rbeg) / PW, nt = nrt * (nrt + 1) / 2;
+      const int i = lane & 15, kq = lane >> 4;
+      for (int e = wave; e < nt; e += 8) {
+        int r0[2], c0[2];
+        bool two = e + 4 < nt;
 #pragma unroll
-        for (int v = 0; v < 4; v++) P[r0 + 4 * v + (lane >> 4)][c0 + (lane & 15)] -= a[v];
+        for (int h = 0; h < 2; h++) {
+          const int et = min(e + 4 * h, nt - 1);
+          int ti = 0;
+          while ((ti + 1) * (ti + 2) / 2 <= et) ti++;
+          r0[h] = rbeg + PW * ti;
+          c0[h] = rbeg + PW * (et - ti * (ti + 1) / 2);
+        }
+        d4 acc[2];
+        double fa[2][4], fb[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[h][v] = P[r0[h] + 4 * v + kq][c0[h] + i];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            fa[h][k] = -P[r0[h] + i][kb + 4 * k + kq];
+            fb[h][k] = P[c0[h] + i][kb + 4 * k + kq];
+          }
+        }
+        // C(r, c) -= sum_k A(r, k) B(c, k): result element v is row 4v + lane/16
+        // of the tile, column lane % 16
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+          for (int h = 0; h < 2; h++)
+            acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[h][k], fb[h][k], acc[h], 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) P[r0[0] + 4 * v + kq][c0[0] + i] = acc[0][v];
+        if (two) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) P[r0[1] + 4 * v + kq][c0[1] + i] = acc[1][v];
+        }
       }
       __syncthreads();
     }
+  }
+  }
+
+  // L_kk back to the factor (the inverse below keeps P's lower part), the
+  // stores draining while the inverse is formed
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    int r, col;
+    const int e = tid + u * 256;
+    lower_entry<BK>(min(e, NL - 1), r, col);
+    const double lv = P[r][col];
+    if (e < NL && r < b) stg<COH>(D + r + static_cast<int64_t>(col) * si.f, lv);
   }
 
   // ---- inverse: the 16x16 diagonal blocks (16 lanes each), then doubling ----
@@ -496,13 +577,28 @@ __device__ void potrf_block(const CholDev& c, int s, int step, double* smem)
   if constexpr (BK >= 64) inverse_level<BK, 32>(P, wave, lane);
   if constexpr (BK >= 128) inverse_level<BK, 64>(P, wave, lane);
 
-  for (int col = tid / 64; col < b; col += 4)
-    for (int r = (tid & 63) + col; r < b; r += 64) stg<COH>(D + r + static_cast<int64_t>(col) * si.f, P[r][col]);
-  // inverse into the 128 x 128 dinv slot (column-major); entries outside the
-  // b x b lower triangle stay at the zeros written when the slot was created
+  // the inverse into the 128 x 128 dinv slot (column-major; entries outside
+  // the b x b lower triangle stay at the zeros written when the slot was
+  // created), over the folded lower triangle, LDS reads first so they overlap
   double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
-  for (int col = tid / 64; col < b; col += 4)
-    for (int r = (tid & 63) + col; r < b; r += 64) stg<COH>(W + static_cast<int64_t>(col) * BT + r, xinv<BK>(P, r, col));
+  constexpr int SB2 = (PER + 1) / 2;   // entries per batch
+#pragma unroll
+  for (int u0 = 0; u0 < PER; u0 += SB2) {
+    double xv[SB2];
+#pragma unroll
+    for (int u = 0; u < SB2; u++) {
+      int r, col;
+      lower_entry<BK>(min(tid + (u0 + u) * 256, NL - 1), r, col);
+      xv[u] = xinv<BK>(P, r, col);
+    }
+#pragma unroll
+    for (int u = 0; u < SB2; u++) {
+      int r, col;
+      const int e = tid + (u0 + u) * 256;
+      lower_entry<BK>(min(e, NL - 1), r, col);
+      if (u0 + u < PER && e < NL && r < b) stg<COH>(W + static_cast<int64_t>(col) * BT + r, xv[u]);
+    }
+  }
   if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
 }
 

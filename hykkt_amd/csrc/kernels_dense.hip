@@ -97,19 +97,20 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
       for (int j = 0; j < JN; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   }
 
-  // staging assignment: thread t loads k = t/16, rows (t%16)*E .. +E
-  const int sk = t >> 4, sa = (t & 15) * EA, sb = (t & 15) * EB;
+  // staging assignment: thread t loads k = t/16, rows t%16 + 16 q (q < E): the
+  // 16 lanes of a k write 16 consecutive LDS words (no bank conflicts)
+  const int sk = t >> 4, sa = t & 15, sb = t & 15;
   double ra[EA], rb[EB];
   auto gload = [&](int k0) {
     const int kk = k0 + sk;
 #pragma unroll
     for (int q = 0; q < EA; q++) {
-      const int r = sa + q;
+      const int r = sa + 16 * q;
       ra[q] = (kk < g.K && r < g.M) ? ldg<COH>(g.A + kk * g.lda + r) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < EB; q++) {
-      const int r = sb + q;
+      const int r = sb + 16 * q;
       rb[q] = (kk < g.K && r < g.N) ? ldg<COH>(g.B + kk * g.ldb + r) : 0.0;
     }
   };
@@ -117,9 +118,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, double* smem)
   for (int k0 = 0; k0 < g.K; k0 += KC) {
     __syncthreads();   // the previous chunk's fragments have been read
 #pragma unroll
-    for (int q = 0; q < EA; q++) As[sk * LA + sa + q] = g.alpha * ra[q];   // exact: alpha is +-1
+    for (int q = 0; q < EA; q++) As[sk * LA + sa + 16 * q] = g.alpha * ra[q];   // exact: alpha is +-1
 #pragma unroll
-    for (int q = 0; q < EB; q++) Bs[sk * LB + sb + q] = rb[q];
+    for (int q = 0; q < EB; q++) Bs[sk * LB + sb + 16 * q] = rb[q];
     __syncthreads();
     if (k0 + KC < g.K) gload(k0 + KC);
 #pragma unroll

@@ -256,6 +256,39 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(row) * 8u, wso = static_cast<uint32_t>(q * QW) * SB * 8u;
 
+  if constexpr (!PF) {
+    // Forest supernodes are one block wide (nc <= 128): a pivot item applies
+    // the inverted diagonal block, an update item the one L block.  The
+    // item's slice is loaded before its waits.
+    double buf[QW];
+    if (kind == 0) slice_load<QW>(buf, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
+    else slice_load<QW>(buf, lview(0), vo, so, f8, lvalid(0), live);
+    double v = 0.0;
+    if (q == 0 && live) {
+      v = (kind == 0) ? b[si.a + r] : 0.0;
+      const int64_t fr = sd.front_off[s] + r;
+      for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
+        v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
+    }
+    stamp(sd, tid, 1);
+    if (kind == 1) {
+      if (threadIdx.x < SB)
+        yv[threadIdx.x] = (threadIdx.x < si.nc) ? spin_load(y + si.a + threadIdx.x, sd.err) : 0.0;
+    } else if (q == 0) {
+      yv[row] = live ? v : 0.0;
+    }
+    __syncthreads();
+    part[q][row] = slice_dot<QW>(buf, yv + q * QW);
+    __syncthreads();
+    if (q == 0 && live) {
+      const double t = sum_slices<NQ>(part, row);
+      if (kind == 0) xstore(y + si.a + r, t);
+      else xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v - t);
+    }
+    stamp(sd, tid, 3);
+    return;
+  }
+
   // Both slices are loaded up front: a load still in flight when the wave
   // polls for a value delays the poll (loads complete in order).
   double l[PF ? QW : 1], w[PF ? QW : 1];

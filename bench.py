@@ -176,7 +176,7 @@ def main():
         """K executes over the next systems of the sequence between two device
         synchronizations; with `profile`, HIP events bracket every factor and
         solve launch (a few microseconds each, so this pass runs slower)."""
-        r = dict(lat=[], worst=0.0, iters=[], tf=[], tcg=[], nref=[], first=0.0)
+        r = dict(lat=[], worst=0.0, iters=[], tf=[], tcg=[], nref=[], first=0.0, firsts=[])
         if profile:
             s.profile(True)
         H.device_synchronize()
@@ -193,6 +193,7 @@ def main():
             r["tcg"].append(st["t_cg_ms"])
             r["nref"].append(st["refine_steps"])
             r["first"] = max(r["first"], st["resid4_first"])
+            r["firsts"].append(st["resid4_first"])
             if rc != 0 or not st["resid4"] <= 1e-8:
                 print(f"bench: step {k} rc={rc} resid4={st['resid4']:.3e}", file=sys.stderr,
                       flush=True)
@@ -235,6 +236,8 @@ def main():
         "cg_ms_mean": float(np.mean(tcg)),
         "refine_steps_mean": float(np.mean(nref)),
         "max_resid4_before_refinement": first,
+        "resid4_before_refinement_quantiles": [float(x) for x in np.percentile(
+            main_pass["firsts"], [0, 25, 50, 75, 100])],
         "roofline": roofline(kt, args.steps),
         "kernel_timing_pass": None if prof_pass is None else {
             "ms_per_step": 1e3 * prof_pass["total"] / args.steps,

@@ -341,13 +341,13 @@ def test_iterative_refinement_gpu():
     assert s.execute() == 0
     st0 = s.stats()
     assert st0["refine_steps"] == 0 and st0["resid4"] > 1e-9
-    s.options(refine_max=3)
+    s.options(refine_max=3, refine_tol=1e-9)
     assert s.execute() == 0
     st = s.stats()
     assert st["refine_steps"] >= 1 and st["resid4"] <= 1e-10
     assert st["resid4_first"] == st0["resid4"]
     ref = O.execute(Hc, sy.Ds.v, Jc, Jd, sy.rx, sy.rs, sy.ry, sy.ryd, sy.gamma,
-                    perm=s.permutation())
+                    perm=s.permutation(), refine_tol=1e-9)
     x = np.concatenate(s.solution())
     xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
     assert rel(x, xo) < 1e-8
@@ -384,7 +384,7 @@ def test_delta_hook_gpu():
     Kd[np.arange(Hc.n), np.arange(Hc.n)] += delta / md ** 2
     assert np.linalg.norm(Kd @ x - rhs) / np.linalg.norm(rhs) <= 1e-9
     assert st["resid4"] > 1e-6   # the unshifted system is not what was solved
-    s.options(delta=1e-8, refine_max=3)
+    s.options(delta=1e-8, refine_max=3, refine_tol=1e-10)
     s.execute()
     st = s.stats()
     assert st["refine_steps"] >= 1 and st["resid4"] <= 1e-9

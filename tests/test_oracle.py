@@ -372,7 +372,7 @@ def test_execute_iterative_refinement():
     c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
     H, Jc, Jd = c(s.H), c(s.Jc), c(s.Jd)
     one = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma, refine_max=0)
-    ref = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma)
+    ref = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma, refine_tol=1e-9)
     assert one["stats"]["refine_steps"] == 0
     assert one["stats"]["resid4"] == one["stats"]["resid4_first"] > 1e-9
     assert ref["stats"]["resid4_first"] == one["stats"]["resid4"]
@@ -413,5 +413,5 @@ def test_delta_hook_solves_shifted_system():
         plain = np.linalg.norm(K @ x - rhs) / np.linalg.norm(rhs)
         assert shifted <= 1e-9 and plain > 1e3 * shifted, (delta, shifted, plain)
         assert np.linalg.norm(x - np.linalg.solve(Kd, rhs)) <= 1e-6 * np.linalg.norm(x)
-    refined = O.execute(*args, delta=1e-8)
+    refined = O.execute(*args, delta=1e-8, refine_tol=1e-10)
     assert refined["stats"]["refine_steps"] >= 1 and refined["stats"]["resid4"] <= 1e-9

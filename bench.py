@@ -98,6 +98,35 @@ def cpu_baseline(g, perm, nsample):
             "max_resid4": worst}
 
 
+def served_vs_plain(rate, seconds, npat=16, budget=200e9):
+    """The served form (continuous batching over 16 resident patterns of 1k,
+    5k and 20k buses, 6:3:1) and the plain path (one request at a time on the
+    same resident solvers) on the same Poisson request stream.  Both are over
+    capacity at 2,000 requests/s: the figures are saturation throughput and
+    queueing latency over `seconds`."""
+    from hykkt_amd.served import Server, make_patterns, poisson_requests, summarize
+    t0 = time.perf_counter()
+    pats = make_patterns(npat, seed=100)
+    srv = Server(pats, hbm_budget=budget)
+    setup = time.perf_counter() - t0
+    reqs = poisson_requests(pats, rate, seconds, seed=7, served=srv.served)
+    res = {}
+    for mode in ("served", "plain"):
+        res[mode] = summarize(*srv.run(reqs, plain=(mode == "plain"), deadline=seconds),
+                              patterns=pats)
+        print(f"bench: {mode} stream: {res[mode]}", file=sys.stderr, flush=True)
+    srv.close()
+    for p in pats:
+        p.free()
+    sp, pp = res["served"]["solves_per_s"], res["plain"]["solves_per_s"]
+    return {"stream": f"Poisson {rate:.0f} requests/s for {seconds:.0f} s over {npat} resident "
+                      "patterns (1k/5k/20k buses, 6:3:1), fresh values and rhs per request, "
+                      "gamma 1e5; requests past the window stay pending",
+            "patterns_served": len(srv.served), "hbm_used_gb": srv.hbm_used / 1e9,
+            "setup_s": setup, "served": res["served"], "plain": res["plain"],
+            "served_over_plain": sp / pp if pp > 0 else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +137,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3,
                     help="solves of the sequence the CPU oracle is timed on")
     ap.add_argument("--kernel-timing", type=int, default=1)
+    ap.add_argument("--served-seconds", type=float, default=6.0,
+                    help="length of the served and plain request streams (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=2000.0,
+                    help="Poisson arrival rate of the served stream (requests/s)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (3 times), the "
                          "known-size reference for HBM byte counters")
@@ -206,6 +239,7 @@ def main():
     # the measured pass runs the plain path with nothing added; the per-kernel
     # figures come from a second pass over the same systems with event timing
     main_pass = timed_pass(False)
+    perm0 = s.permutation()
     prof_pass = timed_pass(True) if args.kernel_timing else None
     kt = prof_pass["kt"] if prof_pass else {}
     lat, worst, iters, tf, tcg, nref, first, total = (
@@ -250,10 +284,15 @@ def main():
                         "algorithmic_flops_per_launch": v["flops"] / max(v["launches"], 1),
                         "algorithmic_bytes_per_launch": v["bytes"] / max(v["launches"], 1)}
                     for k, v in kt.items()},
+        "served": None,
         "cpu_baseline": None,
     }
+    if args.served_seconds > 0:
+        s.close()
+        out["served"] = served_vs_plain(args.served_rate, args.served_seconds)
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(g, s.permutation(), args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(g, s.permutation() if s._h.value else perm0,
+                                           args.cpu_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -103,6 +103,7 @@ _SIGS = {
     "hykkt_memcpy_dtoh": ([_vp, _vp, C.c_ulonglong], C.c_int),
     "hykkt_device_synchronize": ([], C.c_int),
     "hykkt_device_copy_f64": ([_vp, _vp, C.c_longlong], C.c_int),
+    "hykkt_device_mem_info": ([C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)], C.c_int),
     "hykkt_mm_read_csr": ([C.c_char_p, C.c_int, C.c_int, _ip, _ip, _ip, _vp, _vp, _vp], C.c_int),
     "hykkt_mm_read_rhs": ([C.c_char_p, _ip, _vp], C.c_int),
     "hykkt_profile_enable": ([_vp, C.c_int], C.c_int),
@@ -347,6 +348,13 @@ class DeviceArray:
 
 def device_synchronize():
     _check(lib().hykkt_device_synchronize())
+
+
+def device_mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_ulonglong(), C.c_ulonglong()
+    _check(lib().hykkt_device_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def device_copy(dst: "DeviceArray", src: "DeviceArray"):

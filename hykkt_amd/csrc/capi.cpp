@@ -359,6 +359,18 @@ int hykkt_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes)
   });
 }
 
+int hykkt_device_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes)
+{
+  return guard([&] {
+    HYKKT_REQUIRE(free_bytes && total_bytes, HYKKT_E_ARG, "null output");
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return HYKKT_OK;
+  });
+}
+
 int hykkt_device_synchronize(void)
 {
   return guard([&] {

@@ -760,13 +760,14 @@ static size_t potrf_smem()
 template <int BK>
 static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st)
 {
-  static bool attr = false;
-  if (!attr) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_potrf_inv<BK>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(potrf_smem<BK>())));
-    attr = true;
-  }
+  // set once per process; a function-local static initialises thread-safely
+  static const bool attr = [] {
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_potrf_inv<BK>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(potrf_smem<BK>())));
+    return true;
+  }();
+  (void)attr;
   k_potrf_inv<BK><<<nsn, 256, potrf_smem<BK>(), st>>>(c, sns, step);
   HIPCHK(hipGetLastError());
 }
@@ -776,13 +777,14 @@ size_t factor_flow_smem() { return std::max(potrf_smem<128>(), dense_nt_smem());
 void factor_flow(const CholDev& c, const FlowDev& f, int grid, hipStream_t st)
 {
   if (f.ntasks <= 0) return;
-  static bool attr = false;
-  if (!attr) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_flow),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(factor_flow_smem())));
-    attr = true;
-  }
+  // set once per process; a function-local static initialises thread-safely
+  static const bool attr = [] {
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_flow),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(factor_flow_smem())));
+    return true;
+  }();
+  (void)attr;
   k_factor_flow<<<std::min(grid, f.ntasks), 256, factor_flow_smem(), st>>>(c, f);
   HIPCHK(hipGetLastError());
 }

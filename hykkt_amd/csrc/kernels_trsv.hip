@@ -631,12 +631,13 @@ void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, doubl
 {
   if (part == 1 && sd.n_bwd_top > 0) {
     constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);   // staged inverse block
-    static bool attr = false;
-    if (!attr) {
-      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(WL)));
-      attr = true;
-    }
+    // set once per process; a function-local static initialises thread-safely
+    static const bool attr = [] {
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(WL)));
+      return true;
+    }();
+    (void)attr;
     k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top, y, x);
   }
   if (part == 0 && sd.n_bwd > 0)

@@ -193,6 +193,10 @@ int hykkt_device_synchronize(void);
 /* dst[i] = src[i], i < count, by a plain streaming copy kernel (null stream):
  * the known-size reference against which HBM byte counters are scaled. */
 int hykkt_device_copy_f64(double* dst, const double* src, long long count);
+/* Free and total device memory in bytes (hipMemGetInfo); the served form's
+ * HBM accounting reads it around each slot it creates.  No counterpart in the
+ * reference. */
+int hykkt_device_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
 
 /* Per-kernel timing of the factorization and triangular-solve launches
  * (no counterpart in the reference, which times whole phases with

@@ -22,6 +22,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Hardware queues for this process's streams: the served stream runs one
+# solver slot (two streams) per resident pattern concurrently; with HIP's
+# default of 4 queues, slots share queues and their kernels serialise.  Set
+# before the HIP runtime starts; the plain path's throughput is the same
+# with 4 or 16.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 
 def metric_name():
@@ -113,18 +119,21 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9):
     res = {}
     for mode in ("served", "plain"):
         res[mode] = summarize(*srv.run(reqs, plain=(mode == "plain"), deadline=seconds),
-                              patterns=pats)
+                              patterns=pats, solo_s=srv.solo_s)
         print(f"bench: {mode} stream: {res[mode]}", file=sys.stderr, flush=True)
     srv.close()
     for p in pats:
         p.free()
     sp, pp = res["served"]["solves_per_s"], res["plain"]["solves_per_s"]
+    sw, pw = res["served"]["solo_s_per_s"], res["plain"]["solo_s_per_s"]
     return {"stream": f"Poisson {rate:.0f} requests/s for {seconds:.0f} s over {npat} resident "
                       "patterns (1k/5k/20k buses, 6:3:1), fresh values and rhs per request, "
                       "gamma 1e5; requests past the window stay pending",
             "patterns_served": len(srv.served), "hbm_used_gb": srv.hbm_used / 1e9,
             "setup_s": setup, "served": res["served"], "plain": res["plain"],
-            "served_over_plain": sp / pp if pp > 0 else None}
+            "solo_ms": {str(pats[p].nbus) + "/" + str(p): 1e3 * t for p, t in srv.solo_s.items()},
+            "served_over_plain": sp / pp if pp > 0 else None,
+            "served_over_plain_solo_weighted": sw / pw if pw > 0 else None}
 
 
 def main():

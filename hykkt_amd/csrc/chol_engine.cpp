@@ -370,7 +370,7 @@ void CholEngine::build_solve()
   bpart_off_.upload(bpo);
   bpart_.alloc(static_cast<size_t>(std::max<int64_t>(bpo[ns], 1)));
   ybuf_.alloc(static_cast<size_t>(std::max(pl_.n, 1)));
-  flags_.alloc(1);
+  flags_.alloc(5);   // wait-bound error word, four sweep item counters
   sd_.fwd_items = fwd_items_.p;
   sd_.bwd_items = bwd_items_.p;
   sd_.fwd_items_top = fwd_top_items_.p;
@@ -383,6 +383,7 @@ void CholEngine::build_solve()
   sd_.bpart = bpart_.p;
   sd_.bpart_off = bpart_off_.p;
   sd_.err = flags_.p;
+  sd_.ticket = flags_.p + 1;
   sd_.trace = nullptr;
   n_bpart_ = bpo[ns];
   if (const char* tp = std::getenv("HYKKT_SOLVE_TRACE")) {
@@ -539,7 +540,7 @@ void CholEngine::solve(double* x)
       pf->end(kind, st_, 2.0 * solve_lower_[part], 8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]));
   };
   // forward: b in x -> y in ybuf; backward: y in ybuf -> x
-  dev::fill_sentinel(ybuf_.p, pl_.n, st_);
+  dev::fill_sentinel(ybuf_.p, pl_.n, st_, sd_.ticket);
   dev::fill_sentinel(ubuf_.p, nu, st_);
   sweep(PK_SOLVE_FWD_FOREST, 0, true);
   sweep(PK_SOLVE_FWD_TOP, 1, true);

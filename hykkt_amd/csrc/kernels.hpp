@@ -177,12 +177,14 @@ struct SolveDev {
   double* bpart;            // backward chunk partials: chunk u of s at bpart_off[s] + u*nc
   const int64_t* bpart_off;
   int* err;                 // 1: a wait hit its bound (sticky until the host clears it)
+  int* ticket;              // 4 item counters (forward forest, forward top, backward top,
+                            // backward forest), zeroed with the first sentinel fill of a solve
   // Development timeline (null normally): 4 wall-clock stamps per item, items
   // of the forward forest, forward top, backward top, backward forest in turn.
   long long* trace;
 };
-// p[i] = the sentinel, i < n
-void fill_sentinel(double* p, int64_t n, hipStream_t st);
+// p[i] = the sentinel, i < n; zero4 (when not null): four ints set to 0
+void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4 = nullptr);
 // L y = b (b, y internal order; y sentinel-filled, ubuf sentinel-filled).
 // part 0: the forest items, part 1: the top items.
 void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,

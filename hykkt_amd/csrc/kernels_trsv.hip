@@ -25,12 +25,13 @@
 // last wait, are loaded into registers before the item waits, so a link of
 // the dependency chain costs the flag round trip and arithmetic only.
 //
-// Items are dispatched in an order where every item depends only on items
-// with a smaller index.  Each XCD dispatches its share of the workgroups in
-// increasing index order, so the smallest unfinished item is always resident
-// and runnable: the sweep cannot deadlock.  Every wait is additionally
-// bounded (SPIN_LIMIT); on expiry the item sets an error word the host
-// reports, and carries on, so the kernel always terminates.
+// Items are numbered so that every item depends only on items with a
+// smaller number, and a workgroup takes its item from a counter when it
+// starts running (take_item), so the smallest unfinished item is always
+// held by a running workgroup: the sweep cannot deadlock, also while other
+// streams' kernels share the device.  Every wait is additionally bounded
+// (SPIN_LIMIT); on expiry the item sets an error word the host reports, and
+// carries on, so the kernel always terminates.
 //
 // Synchronisation is by the data itself.  Every value passed between items
 // (the forward y blocks, update vectors, backward chunk partials, backward x
@@ -126,6 +127,21 @@ __device__ double spin_load(const double* p, int* err)
   return v;
 }
 
+// The launch's next item, in the order workgroups start running.  Item i
+// goes to a workgroup only after items 0..i-1 went to workgroups that were
+// already running, so the smallest unfinished item is always held by a
+// running workgroup whose producers have all finished: the sweep progresses
+// whatever else shares the device (kernels of other streams may hold the
+// compute units, and workgroup dispatch order alone would not guarantee it).
+__device__ __forceinline__ int take_item(int* ticket)
+{
+  __shared__ int item;
+  if (threadIdx.x == 0)
+    item = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return uni(item);
+}
+
 // Stamp k of item `item` in the development timeline (when one is bound).
 __device__ __forceinline__ void stamp(const SolveDev& sd, int item, int k)
 {
@@ -162,8 +178,10 @@ __device__ __forceinline__ double ld(rsrc_t r, uint32_t voff, uint32_t soff)
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n)
+__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n, int* __restrict__ zero4)
 {
+  if (zero4 && blockIdx.x == 0 && threadIdx.x < 4)
+    __hip_atomic_store(zero4 + threadIdx.x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = __builtin_bit_cast(double, SENTINEL);
@@ -227,10 +245,11 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double yv[SB];
   __shared__ double part[NQ][SB];
-  const int s = uni(items[3 * blockIdx.x]);
-  const int kind = uni(items[3 * blockIdx.x + 1]);
-  const int blk = uni(items[3 * blockIdx.x + 2]);
-  const int tid = (PF ? sd.n_fwd : 0) + blockIdx.x;   // timeline slot
+  const int it = take_item(sd.ticket + (PF ? 1 : 0));
+  const int s = uni(items[3 * it]);
+  const int kind = uni(items[3 * it + 1]);
+  const int blk = uni(items[3 * it + 2]);
+  const int tid = (PF ? sd.n_fwd : 0) + it;   // timeline slot
   stamp(sd, tid, 0);
   const SnInfo si = sninfo(c, s);
   const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
@@ -501,10 +520,11 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double xs[SB];
   __shared__ double part[NQ][SB];
-  const int s = uni(items[3 * blockIdx.x]);
-  const int kind = uni(items[3 * blockIdx.x + 1]);
-  const int idx = uni(items[3 * blockIdx.x + 2]);
-  const int tid = sd.n_fwd + sd.n_fwd_top + (PF ? 0 : sd.n_bwd_top) + blockIdx.x;   // timeline slot
+  const int it = take_item(sd.ticket + (PF ? 2 : 3));
+  const int s = uni(items[3 * it]);
+  const int kind = uni(items[3 * it + 1]);
+  const int idx = uni(items[3 * it + 2]);
+  const int tid = sd.n_fwd + sd.n_fwd_top + (PF ? 0 : sd.n_bwd_top) + it;   // timeline slot
   stamp(sd, tid, 0);
   if constexpr (PF) {
     extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS)
@@ -607,12 +627,13 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
 
 }  // namespace
 
-void fill_sentinel(double* p, int64_t n, hipStream_t st)
+void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4)
 {
-  if (n <= 0) return;
+  if (n <= 0 && !zero4) return;
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
-  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p, n);
+  if (g < 1) g = 1;
+  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p, n, zero4);
   HIPCHK(hipGetLastError());
 }
 

@@ -123,6 +123,7 @@ class Server:
         self.keep = keep_solutions
         self.queue_limit = queue_limit
         self.slots = {p.pid: [] for p in patterns}
+        self.failures = []   # messages of solves the library reported as failed
         self.hbm_used = 0.0
         order = sorted(patterns, key=lambda p: -p.nbus)
         for _ in range(slots_per_pattern):
@@ -141,6 +142,15 @@ class Server:
                 self.hbm_used += used
                 self.slots[p.pid].append(s)
         self.served = [p.pid for p in patterns if self.slots[p.pid]]
+        # each pattern's solve time alone (second solve on its first slot,
+        # pool entry 1): the unit for weighting completions by their work
+        self.solo_s = {}
+        for pid in self.served:
+            s = self.slots[pid][0]
+            self.patterns[pid].bind(s, 1 % len(self.patterns[pid].pool))
+            t = time.perf_counter()
+            s.execute()
+            self.solo_s[pid] = time.perf_counter() - t
 
     def close(self):
         for ss in self.slots.values():
@@ -150,9 +160,16 @@ class Server:
 
     # ---- one run over a request list ----
     def _solve(self, s, req: Request) -> Result:
+        """One request on slot `s`.  A solve the library reports as failed
+        (an error code) is a result with that code and an infinite residual;
+        the slot goes on with the next request."""
         p = self.patterns[req.pattern]
-        p.bind(s, req.k)
-        rc = s.execute()
+        try:
+            p.bind(s, req.k)
+            rc = s.execute()
+        except H.HykktError as e:
+            self.failures.append(str(e))
+            return Result(req.rid, req.pattern, 0.0, e.code, float("inf"), None)
         st = s.stats()
         sol = tuple(np.copy(a) for a in s.solution()) if self.keep else None
         return Result(req.rid, req.pattern, 0.0, rc, st["resid4"], sol)
@@ -244,17 +261,23 @@ def poisson_requests(patterns, rate: float, seconds: float, seed: int = 0, pool:
     return reqs
 
 
-def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None):
+def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=None):
     """Counts, throughput (results within `tol` per wall second) and latency
-    quantiles; with `patterns`, also per bus count."""
+    quantiles; with `patterns`, also per bus count.  With `solo_s` (seconds
+    of each pattern's solve run alone), also the solo seconds of the results
+    within `tol` per wall second: a throughput that does not depend on which
+    sizes were completed (1.0 for one solve at a time, back to back)."""
     lat = np.array([r.latency_s for r in results]) * 1e3
     ok = [r for r in results if r.rc == 0 and r.resid4 <= tol]
-    out = {"completed": len(results), "within_tol": len(ok), "rejected": rejected,
+    out = {"completed": len(results), "within_tol": len(ok),
+           "failed": sum(1 for r in results if r.rc != 0), "rejected": rejected,
            "pending": pending, "wall_s": wall,
            "solves_per_s": len(ok) / wall if wall > 0 else 0.0,
            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
            "max_resid4": max((r.resid4 for r in results), default=None)}
+    if solo_s is not None:
+        out["solo_s_per_s"] = sum(solo_s[r.pattern] for r in ok) / wall if wall > 0 else 0.0
     if patterns is not None:
         by = {}
         for n in sorted({p.nbus for p in patterns}):

@@ -26,6 +26,10 @@ constexpr int SR = 32;    // rows of a short tile
 // next-column update in short tiles: four times the workgroups, each with a
 // quarter of the work, for launches that sit on the critical path.
 constexpr int SHORT_MAX = 128;
+// Two steps share one trailing update with K = 2 * NB (each C tile read and
+// written once for both) while that update has at least this many 128 x 128
+// tiles beyond the two block columns next in line.
+constexpr int PAIR_MIN = 512;
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
@@ -110,6 +114,8 @@ void CholEngine::build_tasks()
   const char* fe = std::getenv("HYKKT_FACTOR_FLOW");
   const bool use_flow = fe && fe[0] == '1';
   if (const char* sp = std::getenv("HYKKT_FLOW_SPIN")) flow_spin_ = std::max(1, std::atoi(sp));
+  int pair_min = PAIR_MIN;
+  if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) pair_min = std::max(1, std::atoi(pm));
   std::vector<int> flow;
   std::vector<int64_t> cnt_off(pl_.ns + 1, 0);
   int64_t ncnt = 0;
@@ -169,7 +175,31 @@ void CholEngine::build_tasks()
     L.trsm.resize(nsteps);
     L.upd_next.resize(nsteps);
     L.upd_rest.resize(nsteps);
+    // Pairing: steps j, j+1 (kinds 1, 2) defer block j's update of the
+    // columns after block j+1 and apply it together with block j+1's, K = 2
+    // NB, while that update is big (its tiles beyond the next two block
+    // columns); kind 0 steps update after their own block.
+    L.kind.assign(nsteps, 0);
+    for (int j = 0; j + 1 < nsteps;) {
+      int64_t rest2 = 0;
+      for (int q = b0; q < b1; q++) {
+        const int s = pl_.level_sn[q];
+        const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
+        if (k0 >= nc) continue;
+        const int cs = k0 + std::min(2 * NB, nc - k0);
+        const int rt = cdiv(f - cs, NB), ct = cdiv(nc - cs, NB);
+        for (int tc = 2; tc < ct; tc++) rest2 += rt - tc;
+      }
+      if (rest2 >= pair_min) {
+        L.kind[j] = 1;
+        L.kind[j + 1] = 2;
+        j += 2;
+      } else {
+        j++;
+      }
+    }
     for (int j = 0; j < nsteps; j++) {
+      const int kind = L.kind[j];
       L.potrf[j].off = static_cast<int>(potrf.size());
       L.trsm[j].off = static_cast<int>(trsm_sn.size());
       int full_tiles = 0;
@@ -180,7 +210,8 @@ void CholEngine::build_tasks()
       }
       const bool shrt = full_tiles <= SHORT_MAX;
       const int rows_unit = shrt ? SR : NB;
-      L.trsm[j].short_rows = L.upd_next[j].short_rows = shrt;
+      L.trsm[j].short_rows = shrt;
+      L.upd_next[j].short_rows = shrt && kind != 2;
       for (int q = b0; q < b1; q++) {
         const int s = pl_.level_sn[q];
         const int nc = pl_.nc(s), f = nc + pl_.nr(s);
@@ -200,13 +231,30 @@ void CholEngine::build_tasks()
         L.trsm[j].flops += r * db * db;
         L.trsm[j].bytes += 8.0 * (2 * r * db + tri);
         // the update splits into the next block column (tile column 0) and
-        // the rest, so the next step can start before the rest is done
-        const double cn = std::min<double>(NB, cc);
-        const double pn = cn * r - cn * (cn - 1) / 2;
-        L.upd_next[j].flops += 2.0 * db * pn;
-        L.upd_next[j].bytes += 8.0 * (r * db + 2 * pn);
-        L.upd_rest[j].flops += 2.0 * db * (pairs - pn);
-        L.upd_rest[j].bytes += 8.0 * 2 * (pairs - pn);
+        // the rest, so the next step can start before the rest is done; a
+        // kind 1 step does the next block column only, the kind 2 step after
+        // it the update by both blocks (K = kw) of the columns after them:
+        // the next two block columns, then the rest
+        if (kind == 2) {
+          const int ka = k0 - NB, kw = std::min(2 * NB, nc - ka);
+          const double dk = kw, r2 = f - (ka + kw), c2 = nc - (ka + kw);
+          const double p2 = c2 * r2 - c2 * (c2 - 1) / 2;
+          const double cn = std::min<double>(2 * NB, c2);
+          const double pn = cn * r2 - cn * (cn - 1) / 2;
+          L.upd_next[j].flops += 2.0 * dk * pn;
+          L.upd_next[j].bytes += 8.0 * (r2 * dk + 2 * pn);
+          L.upd_rest[j].flops += 2.0 * dk * (p2 - pn);
+          L.upd_rest[j].bytes += 8.0 * 2 * (p2 - pn);
+        } else {
+          const double cn = std::min<double>(NB, cc);
+          const double pn = cn * r - cn * (cn - 1) / 2;
+          L.upd_next[j].flops += 2.0 * db * pn;
+          L.upd_next[j].bytes += 8.0 * (r * db + 2 * pn);
+          if (kind == 0) {
+            L.upd_rest[j].flops += 2.0 * db * (pairs - pn);
+            L.upd_rest[j].bytes += 8.0 * 2 * (pairs - pn);
+          }
+        }
         for (int t = 0; t < cdiv(f - (k0 + b), rows_unit); t++) {
           trsm_sn.push_back(s);
           trsm_tr.push_back(t);
@@ -222,6 +270,20 @@ void CholEngine::build_tasks()
           const int s = pl_.level_sn[q];
           const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
           if (k0 >= nc) continue;
+          if (kind == 1 && part == 1) continue;   // deferred to the pair's update
+          if (kind == 2) {
+            // columns after the pair's two blocks, K = both blocks: the next
+            // two block columns (part 0), then the rest (part 1)
+            const int ka = k0 - NB, cs = ka + std::min(2 * NB, nc - ka);
+            const int rtiles = cdiv(f - cs, NB), ctiles = cdiv(nc - cs, NB);
+            for (int tc = part == 0 ? 0 : 2; tc < (part == 0 ? std::min(2, ctiles) : ctiles); tc++)
+              for (int tr = tc; tr < rtiles; tr++) {
+                upd_sn.push_back(s);
+                upd_tr.push_back(tr);
+                upd_tc.push_back(tc);
+              }
+            continue;
+          }
           const int b = std::min(NB, nc - k0);
           const int rtiles = cdiv(f - (k0 + b), NB), ctiles = cdiv(nc - (k0 + b), NB);
           if (part == 0 && shrt) {   // next block column in 32-row tiles
@@ -430,12 +492,13 @@ void CholEngine::factor(double pivot_tol)
                       t_ea_ccol_.p, st_);
     });
     // Blocked steps with look-ahead.  Main stream: diagonal blocks, TRSM,
-    // update of the next block column; second stream: the rest of the
-    // trailing update, which overlaps the next step's diagonal block and
-    // TRSM.  The next-column update of step j writes columns the rest of
-    // step j-1 also writes, so it waits for that (ev_rest_); the rest of
-    // step j waits for step j's TRSM (ev_main_).  Both events are re-recorded
-    // every step after their waits were enqueued.
+    // update of the next block column(s); second stream: the rest of the
+    // trailing update, which overlaps the next steps' diagonal blocks and
+    // TRSMs.  A main-stream update waits for the pending rests (ev_rest_)
+    // only when it writes a block column they write (rest_lo: the lowest
+    // such column); a rest waits for its step's TRSM and next-column update
+    // (ev_main_).  Both events are re-recorded after their waits were
+    // enqueued.
     if (L.use_flow) {
       dev::fill_i(flow_cnt_.p + L.cnt_lo, 0, L.cnt_n, st_);
       dev::fill_i(flow_next_.p, 0, 1, st_);
@@ -446,8 +509,14 @@ void CholEngine::factor(double pivot_tol)
       });
     }
     bool rest_pending = false;
+    int rest_lo = INT_MAX;
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
+      const int kind = L.kind[j];
+      // kind 2: the pair's update (mode 3, from block step-1, K = 2 blocks)
+      // writes block columns step+1 and step+2, its rest step+3 on
+      const int umode = kind == 2 ? 3 : 0, ustep = kind == 2 ? step - 1 : step;
+      const int upd_hi = kind == 2 ? step + 2 : step + 1;
       timed(PK_POTRF, L.potrf[j], [&] {
         dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb,
                        st_);
@@ -457,23 +526,27 @@ void CholEngine::factor(double pivot_tol)
                       t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_,
                       L.trsm[j].short_rows);
       });
-      if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
-      rest_pending = false;
+      if (rest_pending && upd_hi >= rest_lo) {
+        HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
+        rest_pending = false;
+        rest_lo = INT_MAX;
+      }
       timed(PK_DENSE_NT, L.upd_next[j], [&] {
         dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
-                      t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, 0, step,
-                      st_, L.upd_next[j].short_rows);
+                      t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, umode,
+                      ustep, st_, L.upd_next[j].short_rows);
       });
       if (L.upd_rest[j].cnt > 0) {
         HIPCHK(hipEventRecord(ev_main_, st_));
         HIPCHK(hipStreamWaitEvent(st2_, ev_main_, 0));
         timed_on(st2_, PK_DENSE_NT, L.upd_rest[j], [&] {
           dev::dense_nt(cd_, L.upd_rest[j].cnt, t_upd_sn_.p + L.upd_rest[j].off,
-                        t_upd_tr_.p + L.upd_rest[j].off, t_upd_tc_.p + L.upd_rest[j].off, 0, step,
-                        st2_);
+                        t_upd_tr_.p + L.upd_rest[j].off, t_upd_tc_.p + L.upd_rest[j].off, umode,
+                        ustep, st2_);
         });
         HIPCHK(hipEventRecord(ev_rest_, st2_));
         rest_pending = true;
+        rest_lo = std::min(rest_lo, upd_hi + 1);
       }
     }
     if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));

@@ -218,7 +218,7 @@ def main():
         """K executes over the next systems of the sequence between two device
         synchronizations; with `profile`, HIP events bracket every factor and
         solve launch (a few microseconds each, so this pass runs slower)."""
-        r = dict(lat=[], worst=0.0, iters=[], tf=[], tcg=[], nref=[], first=0.0, firsts=[])
+        r = dict(lat=[], worst=0.0, iters=[], tf=[], tcg=[], tref=[], nref=[], first=0.0, firsts=[])
         if profile:
             s.profile(True)
         H.device_synchronize()
@@ -233,6 +233,7 @@ def main():
             r["iters"].append(st["cg_iters"])
             r["tf"].append(st["t_factor_ms"])
             r["tcg"].append(st["t_cg_ms"])
+            r["tref"].append(st["t_refine_ms"])
             r["nref"].append(st["refine_steps"])
             r["first"] = max(r["first"], st["resid4_first"])
             r["firsts"].append(st["resid4_first"])
@@ -278,6 +279,7 @@ def main():
         "factor_ms_mean": float(np.mean(tf)),
         "cg_ms_mean": float(np.mean(tcg)),
         "refine_steps_mean": float(np.mean(nref)),
+        "refine_ms_mean": float(np.mean(main_pass["tref"])),
         "max_resid4_before_refinement": first,
         "resid4_before_refinement_quantiles": [float(x) for x in np.percentile(
             main_pass["firsts"], [0, 25, 50, 75, 100])],

@@ -75,7 +75,7 @@ int hykkt_default_options(hykkt_options* o)
     o->cg_check_every = 1;
     o->verbose = 0;
     o->refine_max = 3;
-    o->refine_tol = 5e-9;   // half the 1e-8 accuracy target
+    o->refine_tol = 9e-9;   // the 1e-8 accuracy target less a 10% margin
     return HYKKT_OK;
   });
 }

@@ -112,7 +112,7 @@ typedef struct {
   int    verbose;           /* print the reference's progress lines (default 0) */
   int    refine_max;        /* iterative-refinement passes on the 4x4 KKT system after the
                                reference's single pass (default 3; 0 = the reference) */
-  double refine_tol;        /* refine while resid4 > refine_tol (default 5e-9); a pass is
+  double refine_tol;        /* refine while resid4 > refine_tol (default 9e-9); a pass is
                                kept only if it lowers resid4 */
 } hykkt_options;
 

@@ -278,7 +278,7 @@ def threads():
 
 def execute(H: Csr, Ds_vals, Jc: Csr, Jd: Csr, rx, rs, ry, ryd, gamma, perm=None, y0=None,
             ruiz_its=2, cg_tol=1e-12, cg_itmax=100, pivot_tol=1e-12, norm_tol=1e-2,
-            intermediates=False, chol_kind=0, refine_max=3, refine_tol=5e-9, delta=0.0):
+            intermediates=False, chol_kind=0, refine_max=3, refine_tol=9e-9, delta=0.0):
     """Oracle execute(); returns dict with dx, ds, dyc, dyd, stats (+ intermediates).
     chol_kind 1 selects the supernodal Cholesky (same factor to rounding);
     refine_max 0 stops after the reference's single pass; delta is added to

@@ -446,6 +446,7 @@ void CholEngine::build_solve()
   sd_.bpart_off = bpart_off_.p;
   sd_.err = flags_.p;
   sd_.ticket = flags_.p + 1;
+  sd_.grid_fwd = n_cu_ * dev::forest_blocks_per_cu();
   sd_.trace = nullptr;
   n_bpart_ = bpo[ns];
   if (const char* tp = std::getenv("HYKKT_SOLVE_TRACE")) {

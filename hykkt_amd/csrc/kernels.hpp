@@ -179,10 +179,13 @@ struct SolveDev {
   int* err;                 // 1: a wait hit its bound (sticky until the host clears it)
   int* ticket;              // 4 item counters (forward forest, forward top, backward top,
                             // backward forest), zeroed with the first sentinel fill of a solve
+  int grid_fwd;             // workgroups of the persistent forward forest sweep (resident at once)
   // Development timeline (null normally): 4 wall-clock stamps per item, items
   // of the forward forest, forward top, backward top, backward forest in turn.
   long long* trace;
 };
+// Workgroups of the forward forest sweep one compute unit holds at once.
+int forest_blocks_per_cu();
 // p[i] = the sentinel, i < n; zero4 (when not null): four ints set to 0
 void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4 = nullptr);
 // L y = b (b, y internal order; y sentinel-filled, ubuf sentinel-filled).

@@ -45,6 +45,8 @@
 // level).  Factor values are plain loads: earlier kernels wrote them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace hykkt {
@@ -233,9 +235,9 @@ __device__ __forceinline__ double slice_dot_stream(rsrc_t rs, uint32_t vo, uint3
 
 // ---------------------------------------------------------------- forward
 // NT threads per item; PF: the L slice of the next step and the inverted
-// diagonal block are prefetched into registers before the waits (top of the
-// tree); otherwise slices are streamed after each wait (small supernodes,
-// high occupancy).
+// diagonal block are prefetched into registers before the waits (the top of
+// the tree, NT_TOP); otherwise slices are streamed after each wait.  The
+// small-supernode forest has its own persistent kernel (k_fwd_forest).
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
@@ -274,39 +276,6 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(row) * 8u, wso = static_cast<uint32_t>(q * QW) * SB * 8u;
-
-  if constexpr (!PF) {
-    // Forest supernodes are one block wide (nc <= 128): a pivot item applies
-    // the inverted diagonal block, an update item the one L block.  The
-    // item's slice is loaded before its waits.
-    double buf[QW];
-    if (kind == 0) slice_load<QW>(buf, wv, wvo, wso, SB * 8u, nrows - q * QW, live);
-    else slice_load<QW>(buf, lview(0), vo, so, f8, lvalid(0), live);
-    double v = 0.0;
-    if (q == 0 && live) {
-      v = (kind == 0) ? b[si.a + r] : 0.0;
-      const int64_t fr = sd.front_off[s] + r;
-      for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
-        v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
-    }
-    stamp(sd, tid, 1);
-    if (kind == 1) {
-      if (threadIdx.x < SB)
-        yv[threadIdx.x] = (threadIdx.x < si.nc) ? spin_load(y + si.a + threadIdx.x, sd.err) : 0.0;
-    } else if (q == 0) {
-      yv[row] = live ? v : 0.0;
-    }
-    __syncthreads();
-    part[q][row] = slice_dot<QW>(buf, yv + q * QW);
-    __syncthreads();
-    if (q == 0 && live) {
-      const double t = sum_slices<NQ>(part, row);
-      if (kind == 0) xstore(y + si.a + r, t);
-      else xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v - t);
-    }
-    stamp(sd, tid, 3);
-    return;
-  }
 
   // Both slices are loaded up front: a load still in flight when the wave
   // polls for a value delays the poll (loads complete in order).
@@ -359,6 +328,93 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     if (q == 0 && live) xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v);
   }
   stamp(sd, tid, 3);
+}
+
+// ------------------------------------------------------- forest, persistent
+// The forward sweep of the small-supernode forest runs on resident
+// workgroups that loop over items.  A workgroup takes its next item as soon as the current one has all
+// its inputs (past its waits): it never holds an item it cannot start, so
+// the smallest unfinished item is still always being worked on, and the
+// counter's round trip overlaps the item's arithmetic.
+__device__ __forceinline__ int ticket_add(int* ticket)
+{
+  return __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forest supernodes are one block wide (nc <= 128): a pivot item applies the
+// inverted diagonal block, an update item the one L block.  The item's slice
+// is loaded before its waits.
+__global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd,
+                                                          const int* __restrict__ items,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ y)
+{
+  constexpr int NQ = NT_FOREST / SB, QW = SB / NQ;
+  __shared__ double yv[SB];
+  __shared__ double part[NQ][SB];
+  __shared__ int nxt;
+  int* const ticket = sd.ticket + 0;
+  if (threadIdx.x == 0) nxt = ticket_add(ticket);
+  __syncthreads();
+  int it = uni(nxt);
+  while (it < sd.n_fwd) {
+    const int s = uni(items[3 * it]);
+    const int kind = uni(items[3 * it + 1]);
+    const int blk = uni(items[3 * it + 2]);
+    const int tid = it;   // timeline slot
+    stamp(sd, tid, 0);
+    const SnInfo si = sninfo(c, s);
+    const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
+    const int rend = (kind == 0) ? min(si.nc, r0 + SB) : min(si.f, r0 + SB);
+    const int nrows = rend - r0;
+    const int row = threadIdx.x & (SB - 1);
+    const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // whole waves per slice
+    const bool live = row < nrows;
+    const int r = r0 + row;
+    const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
+    double buf[QW];
+    if (kind == 0) {
+      // row `row` of the inverted diagonal block (upper part stored as zeros)
+      const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+                             SB * SB * 8u);
+      slice_load<QW>(buf, wv, static_cast<uint32_t>(row) * 8u,
+                     static_cast<uint32_t>(q * QW) * SB * 8u, SB * 8u, nrows - q * QW, live);
+    } else {
+      // row r of the one column block: L(r, q*QW + k)
+      const rsrc_t lv = view(si.Ls, static_cast<uint32_t>(min(SB, si.nc)) * f8);
+      slice_load<QW>(buf, lv, static_cast<uint32_t>(r) * 8u, static_cast<uint32_t>(q * QW) * f8, f8,
+                     min(SB, si.nc) - q * QW, live);
+    }
+    double v = 0.0;
+    if (q == 0 && live) {
+      v = (kind == 0) ? b[si.a + r] : 0.0;
+      const int64_t fr = sd.front_off[s] + r;
+      for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
+        v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
+    }
+    stamp(sd, tid, 1);
+    if (kind == 1) {
+      if (threadIdx.x < SB)
+        yv[threadIdx.x] = (threadIdx.x < si.nc) ? spin_load(y + si.a + threadIdx.x, sd.err) : 0.0;
+    } else if (q == 0) {
+      yv[row] = live ? v : 0.0;
+    }
+    __syncthreads();
+    // every input of this item is in: take the next item now
+    int nx = 0;
+    if (threadIdx.x == 0) nx = ticket_add(ticket);
+    part[q][row] = slice_dot<QW>(buf, yv + q * QW);
+    __syncthreads();
+    if (q == 0 && live) {
+      const double t = sum_slices<NQ>(part, row);
+      if (kind == 0) xstore(y + si.a + r, t);
+      else xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v - t);
+    }
+    stamp(sd, tid, 3);
+    if (threadIdx.x == 0) nxt = nx;
+    __syncthreads();
+    it = uni(nxt);
+  }
 }
 
 // ---------------------------------------------------------------- backward
@@ -637,11 +693,18 @@ void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4)
   HIPCHK(hipGetLastError());
 }
 
+int forest_blocks_per_cu()
+{
+  int nb = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fwd_forest, NT_FOREST, 0));
+  return std::max(nb, 1);
+}
+
 void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,
                    hipStream_t st)
 {
   if (part == 0 && sd.n_fwd > 0)
-    k_solve_fwd<NT_FOREST, false><<<sd.n_fwd, NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, b, y);
+    k_fwd_forest<<<std::min(sd.n_fwd, sd.grid_fwd), NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, b, y);
   if (part == 1 && sd.n_fwd_top > 0)
     k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, b, y);
   HIPCHK(hipGetLastError());

@@ -16,7 +16,7 @@ const char* Profiler::name(int k)
   // kernel names as rocprofv3 prints them (template instances of the sweeps)
   static const char* const names[PK_COUNT] = {
       "k_dense_nt",           "k_potrf_inv",         "k_extend_add",
-      "k_solve_fwd<256, false>", "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
+      "k_fwd_forest",         "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
       "k_solve_bwd<256, false>", "k_factor_flow"};
   return (k >= 0 && k < PK_COUNT) ? names[k] : "";
 }

@@ -86,8 +86,10 @@ def test_kernel_timing_accounting(seq):
     st = s.stats()
     for k, v in kt.items():
         assert v["launches"] > 0 and v["ms"] > 0, k
-    for part in ("<256, false>", "<1024, true>"):   # small-supernode forest, top of the tree
-        assert kt["k_solve_fwd" + part]["launches"] == kt["k_solve_bwd" + part]["launches"] >= 3
+    # small-supernode forest, top of the tree
+    for fwd, bwd in (("k_fwd_forest", "k_solve_bwd<256, false>"),
+                     ("k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>")):
+        assert kt[fwd]["launches"] == kt[bwd]["launches"] >= 3
     fl = kt["k_dense_nt"]["flops"] + kt["k_potrf_inv"]["flops"]
     assert 0.999 <= fl / st["factor_flops"] <= 1.2, fl / st["factor_flops"]
     # event-timed kernels fit inside the solve's own wall span: each kernel's

@@ -136,6 +136,34 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9):
             "served_over_plain_solo_weighted": sw / pw if pw > 0 else None}
 
 
+def batched_vs_plain():
+    """The batched form (256 systems: 32 patterns of 200..5000 buses x 8 value
+    draws, a gamma per system, solved concurrently on one resident slot per
+    pattern) against the same systems one at a time on the same slots."""
+    from hykkt_amd.batched import make_batch, run_batch, summarize_batch
+    from hykkt_amd.served import Server
+    t0 = time.perf_counter()
+    pats = make_batch(seed=3)
+    srv = Server(pats, hbm_budget=200e9)
+    setup = time.perf_counter() - t0
+    if len(srv.served) != len(pats):
+        raise RuntimeError(f"batched: slots refused: {srv.refused}")   # over the budget
+    res = {}
+    for mode in ("batched", "plain"):
+        r, wall = run_batch(srv, pats, plain=(mode == "plain"))
+        res[mode] = summarize_batch(r, wall, pats)
+        print(f"bench: {mode} batch: {res[mode]}", file=sys.stderr, flush=True)
+    srv.close()
+    for p in pats:
+        p.free()
+    return {"batch": "256 systems: 32 patterns (200..5000 buses, branches 1.3x) x 8 value draws, "
+                     "gamma log-uniform 1e4..1e6 per system, Ds log-uniform 1e-4..1e4",
+            "hbm_used_gb": srv.hbm_used / 1e9, "setup_s": setup,
+            "batched": res["batched"], "plain": res["plain"],
+            # systems within the residual bound per second, batched over plain
+            "batched_over_plain": res["batched"]["solves_per_s"] / res["plain"]["solves_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +174,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3,
                     help="solves of the sequence the CPU oracle is timed on")
     ap.add_argument("--kernel-timing", type=int, default=1)
+    ap.add_argument("--batched", type=int, default=1,
+                    help="run the 256-system batch, batched vs one at a time")
     ap.add_argument("--served-seconds", type=float, default=6.0,
                     help="length of the served and plain request streams (0: skip)")
     ap.add_argument("--served-rate", type=float, default=2000.0,
@@ -296,11 +326,15 @@ def main():
                         "algorithmic_bytes_per_launch": v["bytes"] / max(v["launches"], 1)}
                     for k, v in kt.items()},
         "served": None,
+        "batched": None,
         "cpu_baseline": None,
     }
-    if args.served_seconds > 0:
+    if args.served_seconds > 0 or args.batched:
         s.close()
+    if args.served_seconds > 0:
         out["served"] = served_vs_plain(args.served_rate, args.served_seconds)
+    if args.batched:
+        out["batched"] = batched_vs_plain()
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, s.permutation() if s._h.value else perm0,
                                            args.cpu_sample)

@@ -21,6 +21,11 @@ so a result is the same alone and in the mix.
 
 Plain.  One worker takes the requests one at a time in arrival order over all
 patterns, on the same resident solvers: the drop-in path run back to back.
+
+Non-positive pivots.  A system whose H_gamma is not positive definite (H~ not
+positive definite on the null space of Jc) stops the factorization at a
+non-positive pivot, as in the reference; its result carries status 1 and the
+pivot, and it counts as failed.
 """
 from __future__ import annotations
 
@@ -54,15 +59,20 @@ class Result:
     rc: int
     resid4: float
     solution: tuple | None = None
+    k: int = -1          # the pattern's value set
+    pivot: int = -1      # first non-positive pivot of the factorization (-1: none)
 
 
 class ServedPattern:
     """One resident sparsity pattern: its generator, index arrays and a pool
     of value sets in HBM (fresh 1+0.05 N(0,1) perturbations, a new rhs, Ds
-    spread growing with the pool index up to 1e-4..1e8)."""
+    spread growing with the pool index up to 1e-4..1e8, or `ds_ranges[k]`;
+    gamma `gamma`, or `gammas[k]`, for entry k)."""
 
-    def __init__(self, pid: int, nbus: int, seed: int, pool: int = 2, gamma: float = 1e5):
+    def __init__(self, pid: int, nbus: int, seed: int, pool: int = 2, gamma: float = 1e5,
+                 ds_ranges=None, gammas=None):
         self.pid, self.nbus = pid, nbus
+        self.gammas = [float(gamma)] * pool if gammas is None else [float(x) for x in gammas]
         self.g = GridPattern(nbus, int(1.3 * nbus), seed=seed, gamma=gamma)
         g = self.g
         i32 = lambda a: H.DeviceArray(np.asarray(a, np.int32))
@@ -72,7 +82,9 @@ class ServedPattern:
         self.pool = []
         self.host = []
         for k in range(pool):
-            sy = g.system(k, ds_range=ds_spread_schedule(k, pool))
+            sy = g.system(k, ds_range=ds_spread_schedule(k, pool) if ds_ranges is None
+                          else ds_ranges[k])
+            sy.gamma = self.gammas[k]
             f = lambda a: H.DeviceArray(np.asarray(a, np.float64))
             self.pool.append(dict(H=f(sy.H.v), Jc=f(sy.Jc.v), Jd=f(sy.Jd.v), Ds=f(sy.Ds.v),
                                   rx=f(sy.rx), rs=f(sy.rs), ry=f(sy.ry), ryd=f(sy.ryd)))
@@ -80,6 +92,7 @@ class ServedPattern:
 
     def bind(self, s: "H.HykktSolver", k: int):
         g, v, ix = self.g, self.pool[k], self.idx
+        s.set_gamma(self.gammas[k])
         s.set_H(ix["Hrp"], ix["Hci"], v["H"], g.n, g.n, g.H.nnz)
         s.set_Ds(ix["Dsrp"], ix["Dsci"], v["Ds"], g.d, g.d, g.d)
         s.set_Jc(ix["Jcrp"], ix["Jcci"], v["Jc"], g.m, g.n, g.Jc.nnz)
@@ -124,6 +137,7 @@ class Server:
         self.queue_limit = queue_limit
         self.slots = {p.pid: [] for p in patterns}
         self.failures = []   # messages of solves the library reported as failed
+        self.refused = []    # slots not kept: failed first solve or over the budget
         self.hbm_used = 0.0
         order = sorted(patterns, key=lambda p: -p.nbus)
         for _ in range(slots_per_pattern):
@@ -131,12 +145,20 @@ class Server:
                 free0, _ = H.device_mem_info()
                 s = H.HykktSolver(p.g.gamma)
                 s.options(inputs_on_device=1)
+                # symbolic analysis + first solve: the slot's full footprint; a
+                # value set with a non-positive pivot still sizes the slot
                 p.bind(s, 0)
-                rc = s.execute()   # symbolic analysis + first solve: the slot's full footprint
+                rc = s.execute()
+                if rc == H.HYKKT_RESIDUAL_FAIL and s.stats()["pivot"] >= 0:
+                    rc = 0
                 H.device_synchronize()
                 free1, _ = H.device_mem_info()
                 used = max(free0 - free1, 0)
                 if rc != 0 or self.hbm_used + used > hbm_budget:
+                    st = s.stats()
+                    self.refused.append({"pattern": p.pid, "nbus": p.nbus, "rc": rc,
+                                         "bytes": used, "resid2": st["resid2"],
+                                         "resid4": st["resid4"], "pivot": st["pivot"]})
                     s.close()
                     continue
                 self.hbm_used += used
@@ -172,7 +194,7 @@ class Server:
             return Result(req.rid, req.pattern, 0.0, e.code, float("inf"), None)
         st = s.stats()
         sol = tuple(np.copy(a) for a in s.solution()) if self.keep else None
-        return Result(req.rid, req.pattern, 0.0, rc, st["resid4"], sol)
+        return Result(req.rid, req.pattern, 0.0, rc, st["resid4"], sol, req.k, st["pivot"])
 
     def run(self, requests, plain: bool = False, deadline: float | None = None):
         """Serve `requests` (arrival times relative to the start, sorted).
@@ -270,7 +292,8 @@ def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=
     lat = np.array([r.latency_s for r in results]) * 1e3
     ok = [r for r in results if r.rc == 0 and r.resid4 <= tol]
     out = {"completed": len(results), "within_tol": len(ok),
-           "failed": sum(1 for r in results if r.rc != 0), "rejected": rejected,
+           "failed": sum(1 for r in results if r.rc != 0),
+           "non_positive_pivot": sum(1 for r in results if r.pivot >= 0), "rejected": rejected,
            "pending": pending, "wall_s": wall,
            "solves_per_s": len(ok) / wall if wall > 0 else 0.0,
            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,

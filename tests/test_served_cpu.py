@@ -26,3 +26,55 @@ def test_summary_counts():
     assert out["completed"] == 3 and out["within_tol"] == 1
     assert out["rejected"] == 2 and out["pending"] == 3 and out["solves_per_s"] == 0.5
     assert abs(out["p50_ms"] - 20.0) < 1e-9 and out["max_resid4"] == 2e-8
+
+
+class _FakeSolver:
+    """Stands in for a solver slot: draw 1 of a pattern raises a library
+    error, draw 2 meets a non-positive pivot, others solve."""
+
+    def __init__(self):
+        self.k = None
+
+    def execute(self):
+        import hykkt_amd as H
+        if self.k == 1:
+            raise H.HykktError(-5, "planted failure")
+        return 1 if self.k == 2 else 0
+
+    def stats(self):
+        return {"resid4": float("nan") if self.k == 2 else 1e-10, "pivot": 7 if self.k == 2 else -1}
+
+
+class _FakePattern:
+    def __init__(self, pid, nbus):
+        self.pid, self.nbus = pid, nbus
+
+    def bind(self, s, k):
+        s.k = k
+
+
+def _fake_server(npat=3):
+    from hykkt_amd.served import Server
+    srv = object.__new__(Server)
+    srv.patterns = [_FakePattern(i, 1000) for i in range(npat)]
+    srv.slots = {i: [_FakeSolver()] for i in range(npat)}
+    srv.served = list(range(npat))
+    srv.keep, srv.queue_limit, srv.failures, srv.refused = False, 4096, [], []
+    return srv
+
+
+def test_workers_survive_failed_requests():
+    """A request the library rejects (an error code) or that meets a
+    non-positive pivot is a failed result; the slot goes on with the next
+    request, in the served and the plain form alike."""
+    from hykkt_amd.served import Request
+    reqs = [Request(i, i % 3, i % 4, 0.0) for i in range(24)]
+    for plain in (False, True):
+        srv = _fake_server()
+        res, rej, pend, wall = srv.run(reqs, plain=plain)
+        assert rej == 0 and pend == 0 and len(res) == 24
+        out = summarize(res, rej, pend, wall)
+        assert out["failed"] == 12 and out["non_positive_pivot"] == 6 and out["within_tol"] == 12
+        assert len(srv.failures) == 6 and all("planted" in f for f in srv.failures)
+        by = {r.rid: r for r in res}
+        assert all(by[i].rc == (-5 if i % 4 == 1 else 1 if i % 4 == 2 else 0) for i in range(24))

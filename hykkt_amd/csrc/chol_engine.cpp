@@ -613,13 +613,13 @@ void CholEngine::solve(double* x)
     if (pf)
       pf->end(kind, st_, 2.0 * solve_lower_[part], 8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]));
   };
-  // forward: b in x -> y in ybuf; backward: y in ybuf -> x
-  dev::fill_sentinel(ybuf_.p, pl_.n, st_, sd_.ticket);
-  dev::fill_sentinel(ubuf_.p, nu, st_);
+  // forward: b in x -> y in ybuf; backward: y in ybuf -> x.  x holds b until
+  // the forward sweep is done, so it is filled after it; the other buffers
+  // (and the item counters) before it, in one launch.
+  dev::fill_sentinel3(ybuf_.p, pl_.n, ubuf_.p, nu, bpart_.p, n_bpart_, st_, sd_.ticket);
   sweep(PK_SOLVE_FWD_FOREST, 0, true);
   sweep(PK_SOLVE_FWD_TOP, 1, true);
   dev::fill_sentinel(x, pl_.n, st_);
-  dev::fill_sentinel(bpart_.p, n_bpart_, st_);
   sweep(PK_SOLVE_BWD_TOP, 1, false);
   sweep(PK_SOLVE_BWD_FOREST, 0, false);
   if (trace_at_ >= 0) {

@@ -188,6 +188,9 @@ struct SolveDev {
 int forest_blocks_per_cu();
 // p[i] = the sentinel, i < n; zero4 (when not null): four ints set to 0
 void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4 = nullptr);
+// The same for three buffers (p_i[0..n_i)) in one launch.
+void fill_sentinel3(double* p0, int64_t n0, double* p1, int64_t n1, double* p2, int64_t n2,
+                    hipStream_t st, int* zero4 = nullptr);
 // L y = b (b, y internal order; y sentinel-filled, ubuf sentinel-filled).
 // part 0: the forest items, part 1: the top items.
 void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,

@@ -180,13 +180,20 @@ __device__ __forceinline__ double ld(rsrc_t r, uint32_t voff, uint32_t soff)
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-__global__ void k_fill_sentinel(double* __restrict__ p, int64_t n, int* __restrict__ zero4)
+// Up to three buffers in one launch: element i of the concatenation
+// p0[0..n0) p1[0..n1) p2[0..n2) takes the sentinel.
+__global__ void k_fill_sentinel(double* __restrict__ p0, int64_t n0, double* __restrict__ p1,
+                                int64_t n1, double* __restrict__ p2, int64_t n2,
+                                int* __restrict__ zero4)
 {
   if (zero4 && blockIdx.x == 0 && threadIdx.x < 4)
     __hip_atomic_store(zero4 + threadIdx.x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t n = n0 + n1 + n2;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = __builtin_bit_cast(double, SENTINEL);
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double* p = i < n0 ? p0 + i : (i < n0 + n1 ? p1 + (i - n0) : p2 + (i - n0 - n1));
+    *p = __builtin_bit_cast(double, SENTINEL);
+  }
 }
 
 // A slice of QW values, value k at byte offset vo + so0 + k*sstride of the
@@ -685,11 +692,21 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
 
 void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4)
 {
+  fill_sentinel3(p, n, nullptr, 0, nullptr, 0, st, zero4);
+}
+
+void fill_sentinel3(double* p0, int64_t n0, double* p1, int64_t n1, double* p2, int64_t n2,
+                    hipStream_t st, int* zero4)
+{
+  n0 = std::max<int64_t>(n0, 0);
+  n1 = std::max<int64_t>(n1, 0);
+  n2 = std::max<int64_t>(n2, 0);
+  const int64_t n = n0 + n1 + n2;
   if (n <= 0 && !zero4) return;
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
   if (g < 1) g = 1;
-  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p, n, zero4);
+  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p0, n0, p1, n1, p2, n2, zero4);
   HIPCHK(hipGetLastError());
 }
 

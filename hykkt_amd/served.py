@@ -132,7 +132,7 @@ class Server:
 
     def __init__(self, patterns, hbm_budget: float, slots_per_pattern: int = 1,
                  queue_limit: int = 4096, keep_solutions: bool = False):
-        self.patterns = patterns
+        self.patterns = {p.pid: p for p in patterns}
         self.keep = keep_solutions
         self.queue_limit = queue_limit
         self.slots = {p.pid: [] for p in patterns}
@@ -146,9 +146,15 @@ class Server:
                 s = H.HykktSolver(p.g.gamma)
                 s.options(inputs_on_device=1)
                 # symbolic analysis + first solve: the slot's full footprint; a
-                # value set with a non-positive pivot still sizes the slot
+                # value set with a non-positive pivot still sizes the slot, and
+                # a slot the device cannot hold is refused
                 p.bind(s, 0)
-                rc = s.execute()
+                try:
+                    rc = s.execute()
+                except H.HykktError as e:
+                    self.refused.append({"pattern": p.pid, "nbus": p.nbus, "error": str(e)})
+                    s.close()
+                    continue
                 if rc == H.HYKKT_RESIDUAL_FAIL and s.stats()["pivot"] >= 0:
                     rc = 0
                 H.device_synchronize()

@@ -56,7 +56,7 @@ class _FakePattern:
 def _fake_server(npat=3):
     from hykkt_amd.served import Server
     srv = object.__new__(Server)
-    srv.patterns = [_FakePattern(i, 1000) for i in range(npat)]
+    srv.patterns = {i: _FakePattern(i, 1000) for i in range(npat)}
     srv.slots = {i: [_FakeSolver()] for i in range(npat)}
     srv.served = list(range(npat))
     srv.keep, srv.queue_limit, srv.failures, srv.refused = False, 4096, [], []

@@ -26,10 +26,13 @@ constexpr int SR = 32;    // rows of a short tile
 // next-column update in short tiles: four times the workgroups, each with a
 // quarter of the work, for launches that sit on the critical path.
 constexpr int SHORT_MAX = 128;
-// Two steps share one trailing update with K = 2 * NB (each C tile read and
-// written once for both) while that update has at least this many 128 x 128
-// tiles beyond the two block columns next in line.
+// Up to GMAX consecutive steps share one trailing update with K = G * NB
+// (each C tile read and written once for the group) while that update has at
+// least PAIR_MIN * G^2 / 4 tiles of 128 x 128 beyond the G block columns next
+// in line: a group's inner steps update their next block column with K up to
+// (G-1) NB on the critical path, which only a big trailing update repays.
 constexpr int PAIR_MIN = 512;
+constexpr int GMAX = 8;
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
@@ -116,6 +119,8 @@ void CholEngine::build_tasks()
   if (const char* sp = std::getenv("HYKKT_FLOW_SPIN")) flow_spin_ = std::max(1, std::atoi(sp));
   int pair_min = PAIR_MIN;
   if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) pair_min = std::max(1, std::atoi(pm));
+  int gmax = GMAX;
+  if (const char* gm = std::getenv("HYKKT_GROUP_MAX")) gmax = std::clamp(std::atoi(gm), 1, GMAX);
   std::vector<int> flow;
   std::vector<int64_t> cnt_off(pl_.ns + 1, 0);
   int64_t ncnt = 0;
@@ -175,31 +180,37 @@ void CholEngine::build_tasks()
     L.trsm.resize(nsteps);
     L.upd_next.resize(nsteps);
     L.upd_rest.resize(nsteps);
-    // Pairing: steps j, j+1 (kinds 1, 2) defer block j's update of the
-    // columns after block j+1 and apply it together with block j+1's, K = 2
-    // NB, while that update is big (its tiles beyond the next two block
-    // columns); kind 0 steps update after their own block.
-    L.kind.assign(nsteps, 0);
-    for (int j = 0; j + 1 < nsteps;) {
-      int64_t rest2 = 0;
-      for (int q = b0; q < b1; q++) {
-        const int s = pl_.level_sn[q];
-        const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
-        if (k0 >= nc) continue;
-        const int cs = k0 + std::min(2 * NB, nc - k0);
-        const int rt = cdiv(f - cs, NB), ct = cdiv(nc - cs, NB);
-        for (int tc = 2; tc < ct; tc++) rest2 += rt - tc;
+    // Groups: steps a .. a+G-1 defer their blocks' updates of the columns
+    // after block a+G-1 and apply them together at the group's last step, K =
+    // G NB, while that update is big (its tiles beyond the next G block
+    // columns).  Step a+i of a group first updates block column a+i+1 by
+    // blocks a .. a+i (K = (i+1) NB: the same tiles as a single step's next-
+    // column update).  Single steps (G = 1) update after their own block.
+    L.gpos.assign(nsteps, 0);
+    L.gsize.assign(nsteps, 1);
+    for (int j = 0; j < nsteps;) {
+      int G = 1;
+      for (int g = std::min(gmax, nsteps - j); g >= 2 && G == 1; g--) {
+        int64_t rest = 0;
+        for (int q = b0; q < b1; q++) {
+          const int s = pl_.level_sn[q];
+          const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
+          if (k0 >= nc) continue;
+          const int cs = k0 + std::min(g * NB, nc - k0);
+          const int rt = cdiv(f - cs, NB), ct = cdiv(nc - cs, NB);
+          for (int tc = g; tc < ct; tc++) rest += rt - tc;
+        }
+        if (rest >= static_cast<int64_t>(pair_min) * g * g / 4) G = g;
       }
-      if (rest2 >= pair_min) {
-        L.kind[j] = 1;
-        L.kind[j + 1] = 2;
-        j += 2;
-      } else {
-        j++;
+      for (int i = 0; i < G; i++) {
+        L.gpos[j + i] = i;
+        L.gsize[j + i] = G;
       }
+      j += G;
     }
     for (int j = 0; j < nsteps; j++) {
-      const int kind = L.kind[j];
+      const int gi = L.gpos[j], G = L.gsize[j];
+      const bool last = gi == G - 1;   // the group's update of the columns after it
       L.potrf[j].off = static_cast<int>(potrf.size());
       L.trsm[j].off = static_cast<int>(trsm_sn.size());
       int full_tiles = 0;
@@ -211,7 +222,7 @@ void CholEngine::build_tasks()
       const bool shrt = full_tiles <= SHORT_MAX;
       const int rows_unit = shrt ? SR : NB;
       L.trsm[j].short_rows = shrt;
-      L.upd_next[j].short_rows = shrt && kind != 2;
+      L.upd_next[j].short_rows = shrt && !(last && G > 1);
       for (int q = b0; q < b1; q++) {
         const int s = pl_.level_sn[q];
         const int nc = pl_.nc(s), f = nc + pl_.nr(s);
@@ -231,26 +242,28 @@ void CholEngine::build_tasks()
         L.trsm[j].flops += r * db * db;
         L.trsm[j].bytes += 8.0 * (2 * r * db + tri);
         // the update splits into the next block column (tile column 0) and
-        // the rest, so the next step can start before the rest is done; a
-        // kind 1 step does the next block column only, the kind 2 step after
-        // it the update by both blocks (K = kw) of the columns after them:
-        // the next two block columns, then the rest
-        if (kind == 2) {
-          const int ka = k0 - NB, kw = std::min(2 * NB, nc - ka);
+        // the rest, so the next step can start before the rest is done.  In a
+        // group, an inner step updates the next block column by the group's
+        // blocks so far (K = kw) and the last step the columns after the group
+        // by all its blocks: the next G block columns, then the rest
+        const int ka = k0 - gi * NB;   // the group's first column
+        if (last && G > 1) {
+          const int kw = std::min(G * NB, nc - ka);
           const double dk = kw, r2 = f - (ka + kw), c2 = nc - (ka + kw);
           const double p2 = c2 * r2 - c2 * (c2 - 1) / 2;
-          const double cn = std::min<double>(2 * NB, c2);
+          const double cn = std::min<double>(G * NB, c2);
           const double pn = cn * r2 - cn * (cn - 1) / 2;
           L.upd_next[j].flops += 2.0 * dk * pn;
           L.upd_next[j].bytes += 8.0 * (r2 * dk + 2 * pn);
           L.upd_rest[j].flops += 2.0 * dk * (p2 - pn);
           L.upd_rest[j].bytes += 8.0 * 2 * (p2 - pn);
         } else {
+          const double dk = std::min((gi + 1) * NB, nc - ka);
           const double cn = std::min<double>(NB, cc);
           const double pn = cn * r - cn * (cn - 1) / 2;
-          L.upd_next[j].flops += 2.0 * db * pn;
-          L.upd_next[j].bytes += 8.0 * (r * db + 2 * pn);
-          if (kind == 0) {
+          L.upd_next[j].flops += 2.0 * dk * pn;
+          L.upd_next[j].bytes += 8.0 * (r * dk + 2 * pn);
+          if (G == 1) {
             L.upd_rest[j].flops += 2.0 * db * (pairs - pn);
             L.upd_rest[j].bytes += 8.0 * 2 * (pairs - pn);
           }
@@ -270,13 +283,13 @@ void CholEngine::build_tasks()
           const int s = pl_.level_sn[q];
           const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
           if (k0 >= nc) continue;
-          if (kind == 1 && part == 1) continue;   // deferred to the pair's update
-          if (kind == 2) {
-            // columns after the pair's two blocks, K = both blocks: the next
-            // two block columns (part 0), then the rest (part 1)
-            const int ka = k0 - NB, cs = ka + std::min(2 * NB, nc - ka);
+          if (G > 1 && !last && part == 1) continue;   // deferred to the group's update
+          if (G > 1 && last) {
+            // columns after the group's G blocks, K = all of them: the next G
+            // block columns (part 0), then the rest (part 1)
+            const int ka = k0 - gi * NB, cs = ka + std::min(G * NB, nc - ka);
             const int rtiles = cdiv(f - cs, NB), ctiles = cdiv(nc - cs, NB);
-            for (int tc = part == 0 ? 0 : 2; tc < (part == 0 ? std::min(2, ctiles) : ctiles); tc++)
+            for (int tc = part == 0 ? 0 : G; tc < (part == 0 ? std::min(G, ctiles) : ctiles); tc++)
               for (int tr = tc; tr < rtiles; tr++) {
                 upd_sn.push_back(s);
                 upd_tr.push_back(tr);
@@ -513,11 +526,15 @@ void CholEngine::factor(double pivot_tol)
     int rest_lo = INT_MAX;
     for (size_t j = 0; j < L.potrf.size(); j++) {
       const int step = static_cast<int>(j);
-      const int kind = L.kind[j];
-      // kind 2: the pair's update (mode 3, from block step-1, K = 2 blocks)
-      // writes block columns step+1 and step+2, its rest step+3 on
-      const int umode = kind == 2 ? 3 : 0, ustep = kind == 2 ? step - 1 : step;
-      const int upd_hi = kind == 2 ? step + 2 : step + 1;
+      // in a group of G starting at step a: step a+i updates block column
+      // step+1 by blocks a .. a+i (mode i+2 for K = i+1 blocks from block a,
+      // mode 0 for one block); the last step block columns step+1 .. step+G
+      // by all G blocks (mode G+1), its rest step+G+1 on
+      const int gi = L.gpos[j], G = L.gsize[j], a = step - gi;
+      const bool last = gi == G - 1;
+      const int kblk = last ? G : gi + 1;
+      const int umode = kblk == 1 ? 0 : kblk + 1, ustep = a;
+      const int upd_hi = (last && G > 1) ? step + G : step + 1;
       timed(PK_POTRF, L.potrf[j], [&] {
         dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb,
                        st_);
@@ -547,7 +564,7 @@ void CholEngine::factor(double pivot_tol)
         });
         HIPCHK(hipEventRecord(ev_rest_, st2_));
         rest_pending = true;
-        rest_lo = std::min(rest_lo, upd_hi + 1);
+        rest_lo = std::min(rest_lo, upd_hi + 1);   // the rest starts after the columns just updated
       }
     }
     if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));

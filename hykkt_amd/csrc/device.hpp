@@ -101,9 +101,9 @@ class CholEngine {
   struct Level {
     Launch ea_panel, ea_u, syrk;
     std::vector<Launch> potrf, trsm, upd_next, upd_rest;
-    // per step: 0 own trailing update; 1 first of a pair (next block column
-    // only); 2 second of a pair (update by both blocks, K = 2 x 128)
-    std::vector<int> kind;
+    // per step: its position in its group of consecutive steps and the
+    // group's size (1: the step updates after its own block; see build_tasks)
+    std::vector<int> gpos, gsize;
     // dataflow path (levels with multi-step supernodes): one launch over
     // t_flow_[flow.off ..), counters flow_cnt_[cnt_lo, cnt_lo + cnt_n)
     bool use_flow = false;

@@ -185,7 +185,7 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 // mode 0: trailing panel update after block `step`   C = C - A A^T (K = block)
 // mode 1: update matrix U = -L21 L21^T                 (K = nc, U overwritten)
 // mode 2: TRSM L21(block) = A21 * Dinv^T               (K = block, in place)
-// mode 3: as mode 0 after blocks `step` and `step`+1   (K = both blocks)
+// mode m >= 3: as mode 0 after blocks `step` .. `step`+m-2 (K = m-1 blocks)
 // TM = 128: 128x128 tiles (all modes).  TM = 32: 32-row x 128-column tiles
 // (modes 0 and 2), four times as many workgroups for the short launches on
 // the critical path (the TRSM and the next-column update of a step with few
@@ -217,8 +217,8 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
     g.beta = 0.0;
   } else {
     const int k0 = step * BT;
-    // mode 3: the update by blocks step and step+1 together
-    const int b = min(mode == 3 ? 2 * BT : BT, si.nc - k0);
+    // modes 3 and up: the update by blocks step .. step + mode - 2 together
+    const int b = min(mode >= 3 ? (mode - 1) * BT : BT, si.nc - k0);
     const int cs = k0 + b;
     double* panel = si.Ls + static_cast<int64_t>(k0) * si.f;   // columns k0..k0+b
     if (mode != 2) {

@@ -98,3 +98,33 @@ def test_kernel_timing_accounting(seq):
     for k, v in kt.items():
         assert v["ms"] <= st["t_solve_ms"] * 1.01, k
     assert sum(v["ms"] for v in kt.values()) <= 2.0 * st["t_solve_ms"]
+
+
+def test_grouped_updates_bit_identical(seq, monkeypatch):
+    """Grouped trailing updates (consecutive block steps sharing one update
+    with K = G x 128) give every factor entry the same FMAs in the same order
+    as one step at a time, so the solution is bit-identical; the grouping is
+    in effect on this system (fewer dense launches)."""
+    g, _ = seq
+    sy = g.system(19, ds_range=ds_spread_schedule(19, 20))
+    out, launches = {}, {}
+    for gmax in ("1", "default"):
+        if gmax == "1":
+            monkeypatch.setenv("HYKKT_GROUP_MAX", "1")
+        else:
+            monkeypatch.delenv("HYKKT_GROUP_MAX", raising=False)
+        s = H.HykktSolver(g.gamma)
+        try:
+            _bind(s, sy)
+            s.profile(True)
+            assert s.execute() == 0 and s.stats()["resid4"] <= 1e-8
+            launches[gmax] = s.kernel_times()["k_dense_nt"]["launches"]
+            s.profile(False)
+            out[gmax] = [np.copy(a) for a in s.solution()]
+        finally:
+            s.close()
+    print(f"grouped updates: k_dense_nt launches {launches['1']} one step at a time, "
+          f"{launches['default']} grouped", flush=True)
+    assert launches["default"] < launches["1"]
+    for a, b in zip(out["1"], out["default"]):
+        assert np.array_equal(a, b)

@@ -60,7 +60,8 @@ constexpr int SB = 128;                  // item rows / diagonal block size
 // register prefetch; the small-supernode forest below them uses NT_FOREST.
 constexpr int NT_TOP = 1024;
 constexpr int NT_FOREST = 256;
-constexpr int SPIN_LIMIT = 1 << 22;      // polls of ~100 cycles: ~0.2 s per wait
+constexpr int SPIN_LIMIT = 1 << 22;      // polls; each sleeps >= 128 clocks and waits one
+                                         // device-scope load: seconds per wait at most
 
 struct SnInfo {
   int a, nc, nr, f, np, nu;

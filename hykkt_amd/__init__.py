@@ -117,7 +117,6 @@ _SIGS = {
     "hykkt_amd_order": ([C.c_int, _vp, _vp, _vp], C.c_int),
     "hykkt_chol_analyze_host": ([C.c_int, _vp, _vp, C.POINTER(C.c_longlong), _dp, _ip, _ip,
                                  _ip], C.c_int),
-    "hykkt_flow_selfcheck": ([C.c_int, _vp, _vp, C.POINTER(C.c_longlong), _ip], C.c_int),
     "hykkt_chol_create": ([C.c_int, C.c_int, _vp, _vp, C.POINTER(_vp)], C.c_int),
     "hykkt_chol_factor": ([_vp, _vp, C.c_double, _ip], C.c_int),
     "hykkt_chol_solve": ([_vp, _vp, _vp], C.c_int),
@@ -445,16 +444,6 @@ def chol_analyze_host(rp, ci):
                                          C.byref(ns), C.byref(nl), C.byref(mf)))
     return {"nnz_l": nnz.value, "flops": fl.value, "n_supernodes": ns.value,
             "n_levels": nl.value, "max_front": mf.value}
-
-
-def flow_selfcheck(rp, ci):
-    """Replay the dataflow factorization's task lists for the pattern on the
-    host (raises on a task that would wait on a later one); returns the task
-    and level counts."""
-    rp, ci = _i32(rp), _i32(ci)
-    nt, nl = C.c_longlong(), C.c_int()
-    _check(lib().hykkt_flow_selfcheck(len(rp) - 1, _p(rp), _p(ci), C.byref(nt), C.byref(nl)))
-    return {"tasks": nt.value, "levels": nl.value}
 
 
 class Cholesky:

@@ -29,7 +29,7 @@ HIP_SOURCES = {
     "kernels_dense.hip": [],
     "kernels_trsv.hip": [],
 }
-CXX_SOURCES = ["symbolic.cpp", "amd.cpp", "cholplan.cpp", "flowplan.cpp", "mmio.cpp", "chol_engine.cpp",
+CXX_SOURCES = ["symbolic.cpp", "amd.cpp", "cholplan.cpp", "mmio.cpp", "chol_engine.cpp",
                "profiler.cpp",
                "cg.cpp", "solver.cpp", "capi.cpp"]
 

@@ -540,39 +540,6 @@ int hykkt_chol_analyze_host(int n, const int* rp, const int* ci, long long* nnz_
   });
 }
 
-int hykkt_flow_selfcheck(int n, const int* rp, const int* ci, long long* ntasks, int* nlevels)
-{
-  return guard([&] {
-    NEED(rp);
-    hykkt::Pattern a = make_pattern(n, n, rp, ci);
-    hykkt::CholPlan p = hykkt::chol_analyze(a, hykkt::CholOptions{});
-    std::vector<int64_t> cnt_off(p.ns + 1, 0);
-    int64_t ncnt = 0;
-    long long tot = 0;
-    int nl = 0;
-    for (size_t l = 0; l + 1 < p.level_ptr.size(); l++) {
-      const int b0 = p.level_ptr[l], b1 = p.level_ptr[l + 1];
-      int nsteps = 0;
-      for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, (p.nc(p.level_sn[q]) + 127) / 128);
-      if (nsteps < 2) continue;
-      std::vector<int> flow;
-      double fl = 0.0, by = 0.0;
-      const int64_t lo = ncnt;
-      hykkt::flow_tasks(p, b0, b1, cnt_off, ncnt, flow, fl, by);
-      const int ntk = static_cast<int>(flow.size() / 4);
-      const int rc = hykkt::flow_check(p, b0, b1, cnt_off, lo, ncnt - lo, flow.data(), ntk);
-      HYKKT_REQUIRE(rc == 0, HYKKT_E_STATE,
-                    "dataflow task list of level " + std::to_string(l) + " fails its replay (" +
-                        std::to_string(rc) + ")");
-      tot += ntk;
-      nl++;
-    }
-    if (ntasks) *ntasks = tot;
-    if (nlevels) *nlevels = nl;
-    return HYKKT_OK;
-  });
-}
-
 int hykkt_chol_create(int n, int nnz, const int* rp, const int* ci, hykkt_chol* out)
 {
   return guard([&] {

@@ -112,18 +112,10 @@ CholEngine::~CholEngine()
 
 void CholEngine::build_tasks()
 {
-  // The dataflow path is off unless HYKKT_FACTOR_FLOW=1: its first run on the
-  // device did not finish and the cause is not established.
-  const char* fe = std::getenv("HYKKT_FACTOR_FLOW");
-  const bool use_flow = fe && fe[0] == '1';
-  if (const char* sp = std::getenv("HYKKT_FLOW_SPIN")) flow_spin_ = std::max(1, std::atoi(sp));
   int pair_min = PAIR_MIN;
   if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) pair_min = std::max(1, std::atoi(pm));
   int gmax = GMAX;
   if (const char* gm = std::getenv("HYKKT_GROUP_MAX")) gmax = std::clamp(std::atoi(gm), 1, GMAX);
-  std::vector<int> flow;
-  std::vector<int64_t> cnt_off(pl_.ns + 1, 0);
-  int64_t ncnt = 0;
   const int nlev = static_cast<int>(pl_.level_ptr.size()) - 1;
   lv_.assign(nlev, Level{});
   std::vector<int> eap, eacol, eacptr{0}, eacc, eaccol;
@@ -171,11 +163,6 @@ void CholEngine::build_tasks()
     }
     int nsteps = 0;
     for (int q = b0; q < b1; q++) nsteps = std::max(nsteps, cdiv(pl_.nc(pl_.level_sn[q]), NB));
-    if (use_flow && nsteps >= 2) {
-      L.use_flow = true;
-      build_flow(L, b0, b1, flow, cnt_off, ncnt);
-      nsteps = 0;   // no per-step launches
-    }
     L.potrf.resize(nsteps);
     L.trsm.resize(nsteps);
     L.upd_next.resize(nsteps);
@@ -352,25 +339,6 @@ void CholEngine::build_tasks()
   up(t_syrk_sn_, syrk_sn);
   up(t_syrk_tr_, syrk_tr);
   up(t_syrk_tc_, syrk_tc);
-  up(t_flow_, flow);
-  flow_cnt_off_.upload(cnt_off);
-  flow_cnt_.alloc(static_cast<size_t>(std::max<int64_t>(ncnt, 1)));
-  flow_next_.alloc(1);
-  flow_err_.alloc(1);
-  dev::fill_i(flow_err_.p, 0, 1, st_);
-}
-
-void CholEngine::build_flow(Level& L, int b0, int b1, std::vector<int>& flow,
-                            std::vector<int64_t>& cnt_off, int64_t& ncnt)
-{
-  L.flow.off = static_cast<int>(flow.size() / 4);
-  L.cnt_lo = ncnt;
-  double flops = 0.0, bytes = 0.0;
-  flow_tasks(pl_, b0, b1, cnt_off, ncnt, flow, flops, bytes);
-  L.cnt_n = ncnt - L.cnt_lo;
-  L.flow.cnt = static_cast<int>(flow.size() / 4) - L.flow.off;
-  L.flow.flops = flops;
-  L.flow.bytes = bytes;
 }
 
 void CholEngine::build_solve()
@@ -513,15 +481,6 @@ void CholEngine::factor(double pivot_tol)
     // such column); a rest waits for its step's TRSM and next-column update
     // (ev_main_).  Both events are re-recorded after their waits were
     // enqueued.
-    if (L.use_flow) {
-      dev::fill_i(flow_cnt_.p + L.cnt_lo, 0, L.cnt_n, st_);
-      dev::fill_i(flow_next_.p, 0, 1, st_);
-      timed(PK_FLOW, L.flow, [&] {
-        dev::FlowDev fd{t_flow_.p + 4 * static_cast<int64_t>(L.flow.off), L.flow.cnt,
-                        flow_next_.p, flow_cnt_.p, flow_cnt_off_.p, flow_err_.p, flow_spin_};
-        dev::factor_flow(cd_, fd, n_cu_, st_);
-      });
-    }
     bool rest_pending = false;
     int rest_lo = INT_MAX;
     for (size_t j = 0; j < L.potrf.size(); j++) {
@@ -581,27 +540,9 @@ void CholEngine::factor(double pivot_tol)
 
 int CholEngine::pivot()
 {
-  int h = INT_MAX, fe = 0;
+  int h = INT_MAX;
   HIPCHK(hipMemcpyAsync(&h, pivot_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
-  HIPCHK(hipMemcpyAsync(&fe, flow_err_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  if (fe) {
-    // which tiles were short: the first few counters below their final value
-    std::vector<int> cnt(flow_cnt_.n);
-    int nx = 0;
-    HIPCHK(hipMemcpy(cnt.data(), flow_cnt_.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&nx, flow_next_.p, sizeof(int), hipMemcpyDeviceToHost));
-    std::string msg = "factorization: a dataflow wait exceeded its bound (tasks taken " +
-                      std::to_string(nx) + ")";
-    for (const Level& L : lv_) {
-      if (!L.use_flow) continue;
-      msg += "; level counters [" + std::to_string(L.cnt_lo) + ", +" + std::to_string(L.cnt_n) + "]:";
-      for (int64_t i = L.cnt_lo; i < L.cnt_lo + L.cnt_n && i < L.cnt_lo + 24; i++)
-        msg += " " + std::to_string(cnt[i]);
-    }
-    dev::fill_i(flow_err_.p, 0, 1, st_);
-    throw Error(HYKKT_E_HIP, msg);
-  }
   return h == INT_MAX ? -1 : h;
 }
 

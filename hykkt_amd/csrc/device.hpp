@@ -104,17 +104,8 @@ class CholEngine {
     // per step: its position in its group of consecutive steps and the
     // group's size (1: the step updates after its own block; see build_tasks)
     std::vector<int> gpos, gsize;
-    // dataflow path (levels with multi-step supernodes): one launch over
-    // t_flow_[flow.off ..), counters flow_cnt_[cnt_lo, cnt_lo + cnt_n)
-    bool use_flow = false;
-    Launch flow;
-    int64_t cnt_lo = 0, cnt_n = 0;
   };
   void build_tasks();
-  // Tasks of the level's panels in a topological order that favours the
-  // chain of diagonal blocks; appends to flow (4 ints per task).
-  void build_flow(Level& L, int b0, int b1, std::vector<int>& flow, std::vector<int64_t>& cnt_off,
-                  int64_t& ncnt);
   void build_solve();
 
   CholPlan pl_;
@@ -134,10 +125,7 @@ class CholEngine {
   DevBuf<int> t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
   DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tr_, t_trsm_tc_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
   DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
-  DevBuf<int> t_flow_, flow_cnt_, flow_next_, flow_err_;
-  DevBuf<int64_t> flow_cnt_off_;
   int n_cu_ = 1;   // compute units: persistent-launch width
-  int flow_spin_ = 1 << 24;   // dataflow wait bound in polls (seconds)
   // solve items
   DevBuf<int> fwd_items_, bwd_items_, pull_ptr_, pull_src_, flags_;
   DevBuf<int64_t> bpart_off_;

@@ -130,20 +130,6 @@ void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task
 // supernode; max_b = largest block in the launch (selects the LDS footprint).
 // dinv slots must be zero-initialized once (only the b x b lower part is written).
 void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st);
-// Dataflow factorization of one tree level's panels (persistent workgroups,
-// tasks in a topological list order, per-tile operation counters).  tasks: 4
-// ints each (type | short << 2 | quarter << 4, supernode, step, R | C << 16);
-// next and the level's counters are zero before the launch.
-struct FlowDev {
-  const int* tasks;
-  int ntasks;
-  int* next;                // task counter
-  int* cnt;                 // per-tile operation counters (quarter units)
-  const int64_t* cnt_off;   // per supernode: its row-block x column-block counters
-  int* err;                 // 1: a wait hit its bound (the launch still drains)
-  int spin;                 // polls a wait may take before it gives up
-};
-void factor_flow(const CholDev& c, const FlowDev& f, int grid, hipStream_t st);
 
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the

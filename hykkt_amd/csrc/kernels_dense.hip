@@ -615,129 +615,6 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   potrf_block<BK, false>(c, s, step, smem);
 }
 
-// ---------------------------------------------------------------- dataflow
-// Panel factorization of one tree level as tasks over 128 x 128 tiles of each
-// supernode's panel, executed by persistent workgroups in list order (the
-// list is a topological order, so a task only waits for tasks taken before
-// it and the launch cannot deadlock).  Tasks (4 ints): type | short << 2 |
-// quarter << 4, supernode, step k, R | C << 16.
-//   D(k):       diagonal block k (factor + inverse)
-//   T(k, R):    rows of row block R below block k times Dinv_k^T
-//   U(k, R, C): tile (R, C) -= L(R, k) L(C, k)^T
-// Each tile (R, C) of a supernode has a counter of the operations applied to
-// it, in quarter units (a short task covers 32 of the 128 rows and adds 1, a
-// full task adds 4): updates from steps 0..C-1, then D(C) or T(C, R).  A
-// task's inputs are complete when the counters reach their marks; its
-// results are stored coherently and complete (s_waitcnt) before its counter
-// moves.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Wait until *p >= want.  Gives up (sets *err) after `spin` polls or once
-// another wait gave up, so the launch always drains.
-__device__ void flow_wait(const int* p, int want, int* err, int spin)
-{
-  int it = 0;
-  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
-    __builtin_amdgcn_s_sleep(4);
-    ++it;
-    if (it > spin ||
-        ((it & 1023) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
-    }
-  }
-}
-
-// Task bodies, kept out of line so each gets the register file to itself.
-__device__ __noinline__ void flow_diag(const CholDev& c, int s, int k, int b, double* smem)
-{
-  if (b > 64) potrf_block<128, true>(c, s, k, smem);
-  else if (b > 32) potrf_block<64, true>(c, s, k, smem);
-  else potrf_block<32, true>(c, s, k, smem);
-}
-__device__ __noinline__ void flow_gemm(const GemmArgs& g, bool shrt, double* smem)
-{
-  if (shrt) gemm_tile<32, BT, true>(g, smem);
-  else gemm_tile<BT, BT, true>(g, smem);
-}
-
-__global__ void __launch_bounds__(256) k_factor_flow(CholDev c, FlowDev fd)
-{
-  extern __shared__ double smem[];
-  __shared__ int tsk;
-  const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) tsk = atomicAdd(fd.next, 1);
-    __syncthreads();
-    const int t = tsk;
-    __syncthreads();   // every thread has read tsk before it is taken again
-    if (t >= fd.ntasks) return;
-    const int w0 = uni(fd.tasks[4 * t]), s = uni(fd.tasks[4 * t + 1]);
-    const int k = uni(fd.tasks[4 * t + 2]), rc = uni(fd.tasks[4 * t + 3]);
-    const int type = w0 & 3, shrt = (w0 >> 2) & 1, qr = (w0 >> 4) & 3;
-    const int R = rc & 0xffff, C = rc >> 16;
-    const SnInfo si = sninfo(c, s);
-    const int ncb = (si.nc + BT - 1) / BT;
-    int* cnt = fd.cnt + fd.cnt_off[s];
-    auto tile = [&](int r, int cc) { return cnt + r * ncb + cc; };
-    if (tid == 0) {
-      if (type == 0) {
-        flow_wait(tile(k, k), 4 * k, fd.err, fd.spin);
-      } else if (type == 1) {
-        flow_wait(tile(k, k), 4 * k + 4, fd.err, fd.spin);
-        if (R != k) flow_wait(tile(R, k), 4 * k, fd.err, fd.spin);
-      } else {
-        flow_wait(tile(R, k), 4 * k + 4, fd.err, fd.spin);
-        flow_wait(tile(C, k), 4 * k + 4, fd.err, fd.spin);
-        flow_wait(tile(R, C), 4 * k, fd.err, fd.spin);
-      }
-    }
-    __syncthreads();
-    const int k0 = k * BT, b = min(BT, si.nc - k0);
-    if (type == 0) {
-      flow_diag(c, s, k, b, smem);
-    } else {
-      double* panel = si.Ls + static_cast<int64_t>(k0) * si.f;   // columns of block k
-      const int rlo = R * BT + (shrt ? 32 * qr : 0);
-      const int rhi = min(R * BT + (shrt ? 32 * qr + 32 : BT), si.f);
-      GemmArgs g;
-      if (type == 1) {
-        const int R0 = max(rlo, k0 + b);
-        g.A = panel + R0;
-        g.lda = si.f;
-        g.B = c.dinv + c.dinv_off[s] + static_cast<int64_t>(k) * BT * BT;
-        g.ldb = BT;
-        g.C = panel + R0;
-        g.ldc = si.f;
-        g.M = rhi - R0;
-        g.N = b;
-        g.K = b;
-        g.alpha = 1.0;
-        g.beta = 0.0;
-      } else {
-        const int C0 = C * BT;
-        g.A = panel + rlo;
-        g.B = panel + C0;
-        g.lda = g.ldb = si.f;
-        g.C = si.Ls + static_cast<int64_t>(C0) * si.f + rlo;
-        g.ldc = si.f;
-        g.M = rhi - rlo;
-        g.N = min(BT, si.nc - C0);
-        g.K = b;
-        g.alpha = -1.0;
-        g.beta = 1.0;
-      }
-      if (g.M > 0 && g.N > 0) flow_gemm(g, shrt != 0, smem);
-    }
-    __builtin_amdgcn_s_waitcnt(0);   // this thread's results have landed
-    __syncthreads();
-    if (tid == 0) {
-      int* ct = (type == 0) ? tile(k, k) : (type == 1 ? tile(R, k) : tile(R, C));
-      __hip_atomic_fetch_add(ct, shrt ? 1 : 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
 }  // namespace
 
 size_t dense_nt_smem() { return 2 * KC * LDA_S * sizeof(double); }
@@ -771,23 +648,6 @@ static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hi
   }();
   (void)attr;
   k_potrf_inv<BK><<<nsn, 256, potrf_smem<BK>(), st>>>(c, sns, step);
-  HIPCHK(hipGetLastError());
-}
-
-size_t factor_flow_smem() { return std::max(potrf_smem<128>(), dense_nt_smem()); }
-
-void factor_flow(const CholDev& c, const FlowDev& f, int grid, hipStream_t st)
-{
-  if (f.ntasks <= 0) return;
-  // set once per process; a function-local static initialises thread-safely
-  static const bool attr = [] {
-      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_factor_flow),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 static_cast<int>(factor_flow_smem())));
-    return true;
-  }();
-  (void)attr;
-  k_factor_flow<<<std::min(grid, f.ntasks), 256, factor_flow_smem(), st>>>(c, f);
   HIPCHK(hipGetLastError());
 }
 

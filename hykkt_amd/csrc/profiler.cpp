@@ -17,7 +17,7 @@ const char* Profiler::name(int k)
   static const char* const names[PK_COUNT] = {
       "k_dense_nt",           "k_potrf_inv",         "k_extend_add",
       "k_fwd_forest",         "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
-      "k_solve_bwd<256, false>", "k_factor_flow"};
+      "k_solve_bwd<256, false>"};
   return (k >= 0 && k < PK_COUNT) ? names[k] : "";
 }
 

@@ -20,7 +20,6 @@ enum ProfKind {
   PK_SOLVE_FWD_TOP,
   PK_SOLVE_BWD_TOP,
   PK_SOLVE_BWD_FOREST,
-  PK_FLOW,
   PK_COUNT
 };
 

@@ -101,17 +101,4 @@ struct CholOptions {
 // entries with col <= row are used).  a_src of the plan holds CSR positions in a.
 CholPlan chol_analyze(const Pattern& a, const CholOptions& opt);
 
-// Dataflow task list of the panels of supernodes level_sn[b0 .. b1): 4 ints
-// per task (type | short << 2 | quarter << 4, supernode, step, R | C << 16),
-// appended to flow in a topological order; cnt_off[s] = ncnt, then ncnt grows
-// by s's row blocks x column blocks.  flops/bytes: algorithmic totals.
-void flow_tasks(const CholPlan& pl, int b0, int b1, std::vector<int64_t>& cnt_off, int64_t& ncnt,
-                std::vector<int>& flow, double& flops, double& bytes);
-// Replays a task list in order under the counter protocol of the device:
-// 0 if every task's inputs are complete when all earlier tasks are and every
-// tile ends complete; t+1 for the first task that would wait, or a negative
-// code for an incomplete tile.
-int flow_check(const CholPlan& pl, int b0, int b1, const std::vector<int64_t>& cnt_off,
-               int64_t cnt_lo, int64_t cnt_n, const int* tasks, int ntasks);
-
 }  // namespace hykkt

@@ -208,8 +208,6 @@ int hykkt_device_mem_info(unsigned long long* free_bytes, unsigned long long* to
  *   k_potrf_inv   128-column diagonal blocks (b^3/3)
  *   k_extend_add  update-matrix assembly (1 flop, 24 bytes per entry)
  *   k_solve_fwd / k_solve_bwd  one sweep each (2*nnz(L) flops, 8*nnz(L) bytes)
- *   k_factor_flow the dataflow panel factorization of a tree level (off
- *                 unless HYKKT_FACTOR_FLOW=1)
  * hykkt_profile_enable clears the totals; hykkt_profile_get writes up to
  * `capacity` entries, one per kernel launched since, and sets *count to the
  * number of those kernels. */
@@ -255,14 +253,6 @@ int hykkt_amd_order(int n, const int* rp, const int* ci, int* perm);
  * level counts, largest front. */
 int hykkt_chol_analyze_host(int n, const int* rp, const int* ci, long long* nnz_l,
                             double* flops, int* n_supernodes, int* n_levels, int* max_front);
-
-/* Host-only check of the dataflow factorization's task lists for the
- * supernodal plan of an n x n CSR pattern (no counterpart in the reference;
- * for tests): every level with multi-step supernodes gets its task list,
- * which is replayed in order under the device's counter protocol.  Returns
- * HYKKT_E_STATE if a task would wait on a later one or a tile ends
- * incomplete; *ntasks and *nlevels: totals over those levels. */
-int hykkt_flow_selfcheck(int n, const int* rp, const int* ci, long long* ntasks, int* nlevels);
 
 /* CholeskyClass (CholeskyClass.cpp:9-101): sparse Cholesky of the LOWER
  * triangle of an n x n CSR matrix (entries with col <= row). */

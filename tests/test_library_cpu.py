@@ -137,23 +137,3 @@ def test_errors_are_codes_not_exits():
     rp = np.array([0, 1], np.int32)
     r = H.lib().hykkt_amd_order(1, rp.ctypes.data, None, None)
     assert r == -1 and b"null" in H.lib().hykkt_last_error()
-
-
-def test_flow_task_lists_replay():
-    """The dataflow factorization's task lists (host-built) replay in list
-    order under the device's counter protocol: no task waits on a later one
-    and every tile ends complete.  A dense 300-column block plus a sparse
-    tail gives multi-step supernodes with rows below them."""
-    import scipy.sparse as sp
-    rng = np.random.default_rng(4)
-    n = 700
-    A = sp.random(n, n, density=0.004, random_state=5, format="csr")
-    A = A + A.T + sp.eye(n)
-    D = sp.lil_matrix((n, n))
-    D[400:700, 400:700] = np.ones((300, 300))   # dense block -> a 300-column front
-    D[0:400:7, 400:700] = 1.0                   # rows that reach the block
-    D = D.tocsr()
-    P = (abs(A) + abs(D) + abs(D.T)).tocsr()
-    P.sort_indices()
-    r = H.flow_selfcheck(P.indptr, P.indices)
-    assert r["levels"] >= 1 and r["tasks"] > 0, r

@@ -78,30 +78,52 @@ def roofline(kt, steps):
             "share_of_step": v["ms"] / steps}
 
 
-def cpu_baseline(g, perm, nsample):
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    model, count = None, 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("processor"):
+                    count += 1
+    except OSError:
+        pass
+    return model, count
+
+
+def cpu_baseline(g, perm, nsample, gpu_solutions):
     """The CPU oracle (supernodal Cholesky, OpenMP) timed on the first
     `nsample` systems of the same sequence with the same ordering; only its
-    numeric phase counts, as the GPU path's analysis is also done once."""
+    numeric phase counts, as the GPU path's analysis is also done once.  Each
+    oracle solution is also the reference the GPU's solution of that system
+    (gpu_solutions[t]) is measured against."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     from hykkt_amd.generator import ds_spread_schedule
 
     oc = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
-    t_num, worst = 0.0, 0.0
+    t_num, worst, errs = 0.0, 0.0, []
     for t in range(nsample):
         sy = g.system(t, ds_range=ds_spread_schedule(t, 20))
         r = O.execute(oc(sy.H), sy.Ds.v, oc(sy.Jc), oc(sy.Jd), sy.rx, sy.rs, sy.ry, sy.ryd,
                       g.gamma, perm=perm, chol_kind=1)
         t_num += r["stats"]["t_numeric"]
         worst = max(worst, r["stats"]["resid4"])
+        xo = np.concatenate([r["dx"], r["ds"], r["dyc"], r["dyd"]])
+        if t in gpu_solutions:
+            errs.append(float(np.linalg.norm(gpu_solutions[t] - xo) / np.linalg.norm(xo)))
         print(f"bench: cpu oracle solve {t}: {r['stats']['t_numeric']:.2f} s "
-              f"resid4={r['stats']['resid4']:.2e}", file=sys.stderr, flush=True)
+              f"resid4={r['stats']['resid4']:.2e} gpu-vs-oracle="
+              f"{errs[-1] if errs else float('nan'):.2e}", file=sys.stderr, flush=True)
+    model, logical = cpu_model()
     return {"value": nsample / t_num, "unit": "KKT solves/s", "cores": O.threads(),
-            "kind": "port",
+            "kind": "port", "cpu_model": model, "logical_cpus_visible": logical,
             "sample": f"{nsample} solves (steps 0..{nsample - 1}) of the same 20-step sequence, "
                       "oracle/hykkt_oracle.c execute() with its supernodal left-looking "
                       "Cholesky, same ordering, numeric phase only",
-            "max_resid4": worst}
+            "max_resid4": worst, "gpu_err_vs_oracle": errs}
 
 
 def served_vs_plain(rate, seconds, npat=16, budget=200e9):
@@ -281,6 +303,13 @@ def main():
     main_pass = timed_pass(False)
     perm0 = s.permutation()
     prof_pass = timed_pass(True) if args.kernel_timing else None
+    # the GPU's solutions of the systems the CPU oracle solves (its sample)
+    gpu_sol = {}
+    if args.cpu_baseline:
+        for t in range(args.cpu_sample):
+            bind(t)
+            s.execute()
+            gpu_sol[t] = np.concatenate(s.solution())
     kt = prof_pass["kt"] if prof_pass else {}
     lat, worst, iters, tf, tcg, nref, first, total = (
         main_pass[k] for k in ("lat", "worst", "iters", "tf", "tcg", "nref", "first", "total"))
@@ -328,6 +357,7 @@ def main():
         "served": None,
         "batched": None,
         "cpu_baseline": None,
+        "err_vs_oracle": None,
     }
     if args.served_seconds > 0 or args.batched:
         s.close()
@@ -336,8 +366,12 @@ def main():
     if args.batched:
         out["batched"] = batched_vs_plain()
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(g, s.permutation() if s._h.value else perm0,
-                                           args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(g, perm0, args.cpu_sample, gpu_sol)
+        e = out["cpu_baseline"]["gpu_err_vs_oracle"]
+        out["err_vs_oracle"] = {
+            "max_rel": max(e) if e else None,
+            "what": "||x_gpu - x_oracle|| / ||x_oracle|| over (dx, ds, dyc, dyd), same ordering, "
+                    f"on steps 0..{len(e) - 1} of the sequence (the CPU baseline's sample)"}
     print(json.dumps(out), flush=True)
 
 

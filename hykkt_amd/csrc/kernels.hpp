@@ -17,6 +17,8 @@ void fill_i(int* p, int v, int64_t n, hipStream_t st);
 void copy(double* dst, const double* src, int64_t n, hipStream_t st);
 // dst[i] = src[map[i]]  (map_idx, permcheck.cu:12-19)
 void gather(double* dst, const double* src, const int* map, int64_t n, hipStream_t st);
+// dst[i] = src[map[i]] with 64-bit positions
+void gather64(double* dst, const double* src, const int64_t* map, int64_t n, hipStream_t st);
 // dst[i] = src[map[i]] * scale[i]
 void gather_scale(double* dst, const double* src, const int* map, const double* scale, int n,
                   hipStream_t st);
@@ -48,9 +50,11 @@ void add_at(double* out, const int64_t* pos, int n, double delta, hipStream_t st
 
 // ---------------------------------------------------------------- Ruiz
 // adapt_row_max (matrix_vector_ops.cu:218-266). Jt rows read as jt_v[jt_map ? jt_map[q] : q].
+// zero_row (may be null): receives the smallest row index whose entries are
+// all zero (its scale is 1/sqrt(0)) by atomicMin; the caller initialises it.
 void ruiz_rowmax(int n, int totn, const int* h_rp, const double* h_v, const int* jt_rp,
                  const int* jt_map, const double* jt_v, const int* j_rp, const double* j_v,
-                 double* scale, hipStream_t st);
+                 double* scale, hipStream_t st, int* zero_row = nullptr);
 // adapt_diag_scale (matrix_vector_ops.cu:146-186); jt_v_scaled may be null (Jt shares J).
 // first != 0: max_d is written as 1*scale (reset, :268-283, fused).
 void ruiz_scale(int n, int totn, const int* h_rp, const int* h_ci, double* h_v,

@@ -64,6 +64,14 @@ __global__ void k_gather(double* __restrict__ d, const double* __restrict__ s,
     d[i] = s[map[i]];
 }
 
+__global__ void k_gather64(double* __restrict__ d, const double* __restrict__ s,
+                           const int64_t* __restrict__ map, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[map[i]];
+}
+
 __global__ void k_gather_scale(double* __restrict__ d, const double* __restrict__ s,
                                const int* __restrict__ map, const double* __restrict__ sc, int n)
 {
@@ -165,7 +173,7 @@ __global__ void k_ruiz_rowmax(int n, int totn, const int* __restrict__ h_rp,
                               const double* __restrict__ h_v, const int* __restrict__ jt_rp,
                               const int* __restrict__ jt_map, const double* __restrict__ jt_v,
                               const int* __restrict__ j_rp, const double* __restrict__ j_v,
-                              double* __restrict__ scale)
+                              double* __restrict__ scale, int* __restrict__ zero_row)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= totn) return;
@@ -187,6 +195,8 @@ __global__ void k_ruiz_rowmax(int n, int totn, const int* __restrict__ h_rp,
     }
     scale[i] = 1.0 / sqrt(max_l);
   }
+  // a row without a nonzero value (or with a NaN) gets the scale 1/sqrt(0)
+  if (zero_row && !(max_l > 0.0 || max_u > 0.0)) atomicMin(zero_row, i);
 }
 
 __global__ void k_ruiz_scale(int n, int totn, const int* __restrict__ h_rp,
@@ -403,6 +413,15 @@ void gather(double* d, const double* s, const int* map, int64_t n, hipStream_t s
   HIPCHK(hipGetLastError());
 }
 
+void gather64(double* d, const double* s, const int64_t* map, int64_t n, hipStream_t st)
+{
+  if (n <= 0) return;
+  unsigned g = nblocks(n);
+  if (g > 65536) g = 65536;
+  k_gather64<<<g, TPB, 0, st>>>(d, s, map, n);
+  HIPCHK(hipGetLastError());
+}
+
 void gather_scale(double* d, const double* s, const int* map, const double* sc, int n,
                   hipStream_t st)
 {
@@ -472,11 +491,11 @@ void add_at(double* out, const int64_t* pos, int n, double delta, hipStream_t st
 
 void ruiz_rowmax(int n, int totn, const int* h_rp, const double* h_v, const int* jt_rp,
                  const int* jt_map, const double* jt_v, const int* j_rp, const double* j_v,
-                 double* scale, hipStream_t st)
+                 double* scale, hipStream_t st, int* zero_row)
 {
   if (totn <= 0) return;
   k_ruiz_rowmax<<<nblocks(totn), TPB, 0, st>>>(n, totn, h_rp, h_v, jt_rp, jt_map, jt_v, j_rp,
-                                               j_v, scale);
+                                               j_v, scale, zero_row);
   HIPCHK(hipGetLastError());
 }
 

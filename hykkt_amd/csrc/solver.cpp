@@ -41,6 +41,7 @@ struct Solver::Sym {
   // refinement: signed residual rows, scaled rhs, correction / candidate
   DevBuf<double> e_x, e_s, e_y, e_d, e_rxt, e_ysc, e_rxp, e_schur, e_cx;
   DevBuf<double> c_dx, c_dsv, c_dyc, c_dyd;
+  DevBuf<int> zero_row;   // smallest all-zero row seen by the Ruiz sweeps (INT_MAX: none)
   // current value pointers (device)
   const double *hv = nullptr, *jcv = nullptr, *jdv = nullptr, *dsd = nullptr;
   const double *drx = nullptr, *drs = nullptr, *dry = nullptr, *dryd = nullptr;
@@ -318,17 +319,8 @@ void Solver::analyze()
   for (DevBuf<double>* b : {&S->e_x, &S->e_rxt, &S->e_rxp, &S->c_dx}) A(*b, n);
   for (DevBuf<double>* b : {&S->e_y, &S->e_ysc, &S->e_schur, &S->e_cx, &S->c_dyc}) A(*b, m);
   for (DevBuf<double>* b : {&S->e_s, &S->e_d, &S->c_dsv, &S->c_dyd}) A(*b, d);
-  if (!on_dev) {
-    A(S->h_v, S->nnz_h);
-    A(S->jc_v, S->nnz_jc);
-    A(S->jd_v, S->nnz_jd);
-    A(S->ds_v, d);
-    A(S->rx, n);
-    A(S->rs, d);
-    A(S->ry, m);
-    A(S->ryd, d);
-  }
   A(S->y0, m);
+  S->zero_row.alloc(1);
   HIPCHK(hipDeviceSynchronize());
 
   stats.n = n;
@@ -373,8 +365,11 @@ void Solver::upload_values()
     if (vecs_[VEC_Y]) dev::copy(S.y0.p, vecs_[VEC_Y], S.m, st_);
     else dev::fill(S.y0.p, 0.0, S.m, st_);
   } else {
-    // value-only H2D per solve (HykktSolver.cpp:635-666)
+    // value-only H2D per solve (HykktSolver.cpp:635-666), into staging
+    // buffers made on first use (the option inputs_on_device may change
+    // between solves without a new analysis)
     auto h2d = [&](DevBuf<double>& b, const double* src, size_t cnt) {
+      b.ensure(std::max<size_t>(cnt, 1));
       if (cnt) HIPCHK(hipMemcpyAsync(b.p, src, cnt * sizeof(double), hipMemcpyHostToDevice, st_));
     };
     h2d(S.h_v, blk_[BLK_H].v, S.nnz_h);
@@ -385,7 +380,8 @@ void Solver::upload_values()
     h2d(S.rs, vecs_[VEC_RS], S.d);
     h2d(S.ry, vecs_[VEC_RY], S.m);
     h2d(S.ryd, vecs_[VEC_RYD], S.d);
-    if (vecs_[VEC_Y]) h2d(S.y0, vecs_[VEC_Y], S.m);
+    if (vecs_[VEC_Y]) HIPCHK(hipMemcpyAsync(S.y0.p, vecs_[VEC_Y], S.m * sizeof(double),
+                                            hipMemcpyHostToDevice, st_));
     else dev::fill(S.y0.p, 0.0, S.m, st_);
     S.hv = S.h_v.p;
     S.jcv = S.jc_v.p;
@@ -399,6 +395,20 @@ void Solver::upload_values()
 }
 
 void Solver::chol_solve(double* x) { sym_->chol->solve(x); }
+
+// compute_spgemm_hgamma + apply_permutation + add_diag (HykktSolver.cpp:370-395):
+// the lower triangle of P*Hg*P^T, each value evaluated from the Ruiz-scaled
+// H~ and Jc and written straight to its slot in the factor storage, then the
+// diagonal shift.
+void Solver::assemble_hgamma()
+{
+  Sym& S = *sym_;
+  CholEngine& ch = *S.chol;
+  ch.zero_L();
+  dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, S.htil.p,
+                      S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, S.jcs.p, S.jcs.p, gamma_, ch.L(), st_);
+  ch.shift_diag(opts.delta);   // add_diag(delta) hook (HykktSolver.cpp:392)
+}
 
 // ------------------------------------------------------------------ execute
 int Solver::execute()
@@ -433,9 +443,10 @@ int Solver::execute()
 
   // ---- compute_ruiz_scaling (HykktSolver.cpp:340-349; RuizClass.cpp:54-98) ----
   if (opts.ruiz_its <= 0) dev::fill(S.max_d.p, 1.0, n + m, st_);
+  dev::fill_i(S.zero_row.p, INT_MAX, 1, st_);
   for (int it = 0; it < opts.ruiz_its; it++) {
     dev::ruiz_rowmax(n, n + m, S.ht_rp.p, S.htil.p, S.jct_rp.p, S.jct_map.p, S.jcs.p, S.jc_rp.p,
-                     S.jcs.p, S.scale.p, st_);
+                     S.jcs.p, S.scale.p, st_, S.zero_row.p);
     dev::ruiz_scale(n, n + m, S.ht_rp.p, S.ht_ci.p, S.htil.p, S.jct_rp.p, S.jct_ci.p, nullptr,
                     S.jc_rp.p, S.jc_ci.p, S.jcs.p, S.scale.p, S.rx_til.p, S.ry_s.p, S.max_d.p,
                     it == 0, st_);
@@ -443,10 +454,7 @@ int Solver::execute()
 
   // ---- H_gamma straight into the factor storage (HykktSolver.cpp:370-395) ----
   CholEngine& ch = *S.chol;
-  ch.zero_L();
-  dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, S.htil.p,
-                      S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, S.jcs.p, S.jcs.p, gam, ch.L(), st_);
-  ch.shift_diag(opts.delta);   // add_diag(delta) hook (HykktSolver.cpp:392)
+  assemble_hgamma();
 
   // ---- compute_hgamma_factorization (HykktSolver.cpp:398-401) ----
   ch.factor(opts.pivot_tol);
@@ -466,7 +474,16 @@ int Solver::execute()
 
   // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
   double hr[8];
+  int zrow = INT_MAX;
+  HIPCHK(hipMemcpyAsync(&zrow, S.zero_row.p, sizeof(int), hipMemcpyDeviceToHost, st_));
   residual(S.dx.p, S.dsv.p, S.dyc.p, S.dyd.p, true, ev_[3], hr);
+  // matrix_vector_ops.cu:258-264 scales such a row by 1/sqrt(0) = inf and the
+  // solve turns to NaN; here it is an argument error
+  if (zrow != INT_MAX)
+    throw Error(HYKKT_E_ARG, zrow < n ? "row " + std::to_string(zrow) +
+                                            " of [H~ Jc^T] has no nonzero value (Ruiz scale 1/sqrt(0))"
+                                      : "row " + std::to_string(zrow - n) +
+                                            " of Jc has no nonzero value (Ruiz scale 1/sqrt(0))");
   const int hpiv = ch.pivot();
   HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
                 "triangular-solve sweep exceeded its wait bound (results invalid)");
@@ -683,12 +700,27 @@ long long Solver::debug_copy(int what, void* out, long long cap)
     case 7: return dcopy(S.ry_s.p, S.m);
     case 8: return icopy(S.hgp.e.rp);
     case 9: return icopy(S.hgp.e.ci);
-    case 10:   // H_gamma values, natural order, from the current scaled blocks
-      tmp.alloc(std::max(1, S.hgp.e.nnz()));
-      dev::spgemm_values(S.hgp.e.nnz(), S.hg_dsrc.p, S.htil.p, S.hg_tp.p, S.hg_ta.p, S.hg_tb.p,
-                         S.jcs.p, S.jcs.p, gamma_, tmp.p, st_);
+    case 10: {
+      // The assembly of the last execute, run again into the factor storage
+      // (the factor itself is recomputed by every execute), and read back
+      // from there through the assembly map: entry a_src[e] of P*Hg*P^T is
+      // at L[a_dst[e]]; upper-triangle positions are NaN.
+      const long long nz = S.hgp_pat.nnz();
+      HYKKT_REQUIRE(nz <= cap, HYKKT_E_ARG, "debug_copy: buffer too small");
+      CholEngine& ch = *S.chol;
+      const int na = ch.n_assembled();
+      tmp.alloc(std::max(1, na));
+      assemble_hgamma();
+      dev::gather64(tmp.p, ch.L(), ch.a_dst(), na, st_);
+      std::vector<double> got(std::max(1, na));
+      HIPCHK(hipMemcpyAsync(got.data(), tmp.p, sizeof(double) * na, hipMemcpyDeviceToHost, st_));
       HIPCHK(hipStreamSynchronize(st_));
-      return dcopy(tmp.p, S.hgp.e.nnz());
+      double* o = static_cast<double*>(out);
+      std::fill(o, o + nz, std::nan(""));
+      const std::vector<int>& src = ch.plan().a_src;
+      for (int e = 0; e < na; e++) o[src[e]] = got[e];
+      return nz;
+    }
     case 11:   // H~ before Ruiz, recomputed from the current inputs
       tmp.alloc(std::max(1, S.htp.e.nnz()));
       if (S.has_jd)

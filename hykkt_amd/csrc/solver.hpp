@@ -50,6 +50,8 @@ class Solver {
   bool pattern_matches() const;
   void upload_values();
   void chol_solve(double* x);
+  // H_gamma from the Ruiz-scaled blocks into the factor storage (+ delta).
+  void assemble_hgamma();
   // Work buffers of one solve pass (r^x permuted, Schur rhs, CG iterate).
   struct Pass {
     double *rxp, *schur, *cx;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/HykktSolver.hpp"
+#include "solution_dump.hpp"
 
 namespace {
 
@@ -109,6 +110,7 @@ int main(int argc, char* argv[])
   hykkt_stats st = hs.stats();
   std::printf("system 1: status %d resid2 %.3e resid4 %.3e cg_iters %d\n", status, st.resid2,
               st.resid4, st.cg_iters);
+  if (!dump_solution(hs, "system1")) return EXIT_FAILURE;
   if (status == 1) return 1;
 
   overwrite(a.H, b.H);
@@ -123,5 +125,6 @@ int main(int argc, char* argv[])
   st = hs.stats();
   std::printf("system 2: status %d resid2 %.3e resid4 %.3e cg_iters %d\n", status, st.resid2,
               st.resid4, st.cg_iters);
+  if (!dump_solution(hs, "system2")) return EXIT_FAILURE;
   return status;
 }

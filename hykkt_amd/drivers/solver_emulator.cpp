@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/HykktSolver.hpp"
+#include "solution_dump.hpp"
 
 namespace {
 
@@ -64,6 +65,7 @@ int main(int argc, char* argv[])
                 "solve %.2f ms symbolic %.1f ms\n",
                 q.c_str(), status, st.resid2, st.resid4, st.cg_iters, st.refine_steps,
                 st.t_solve_ms, st.t_symbolic_ms);
+    if (!dump_solution(hs, "sequence_" + q)) return EXIT_FAILURE;
     if (status == 1) return 1;
   }
   return 0;

@@ -172,13 +172,20 @@ class Server:
         self.served = [p.pid for p in patterns if self.slots[p.pid]]
         # each pattern's solve time alone (second solve on its first slot,
         # pool entry 1): the unit for weighting completions by their work
+        # (a solve that fails there keeps the pattern out of the weighting)
         self.solo_s = {}
         for pid in self.served:
             s = self.slots[pid][0]
             self.patterns[pid].bind(s, 1 % len(self.patterns[pid].pool))
             t = time.perf_counter()
-            s.execute()
-            self.solo_s[pid] = time.perf_counter() - t
+            try:
+                rc = s.execute()
+            except H.HykktError as e:
+                self.failures.append(f"solo timing of pattern {pid}: {e}")
+                continue
+            dt = time.perf_counter() - t
+            if rc == 0:
+                self.solo_s[pid] = dt
 
     def close(self):
         for ss in self.slots.values():
@@ -306,7 +313,8 @@ def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=
            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
            "max_resid4": max((r.resid4 for r in results), default=None)}
     if solo_s is not None:
-        out["solo_s_per_s"] = sum(solo_s[r.pattern] for r in ok) / wall if wall > 0 else 0.0
+        out["solo_s_per_s"] = (sum(solo_s[r.pattern] for r in ok if r.pattern in solo_s) / wall
+                               if wall > 0 else 0.0)
     if patterns is not None:
         by = {}
         for n in sorted({p.nbus for p in patterns}):

@@ -162,8 +162,11 @@ int hykkt_set_permutation(hykkt_solver s, const int* perm, int n);
  *   4 max_d (n+m)                        5 Jc values after Ruiz (nnz Jc)
  *   6 r~x after Ruiz (n)                 7 ry after Ruiz (m)
  *   8 H_gamma pattern rowptr (n+1)       9 H_gamma colidx (nnz_hgam)
- *  10 H_gamma values of the lower triangle of P*Hg*P^T, in P*Hg*P^T CSR order,
- *     NaN at upper-triangle positions (nnz_hgam)
+ *  10 H_gamma as the factorization receives it: the last execute's assembly
+ *     (Ruiz-scaled H~ + gamma Jc^T Jc, permuted, plus delta on the diagonal)
+ *     is run again into the factor storage and read back from there, in
+ *     P*Hg*P^T CSR order (selectors 15, 16), NaN at upper-triangle positions
+ *     (nnz_hgam).  The factor is overwritten; the solution stays valid.
  *  11 H~ values before Ruiz (nnz_htil)  12 r^x permuted = rxp (n)
  *  13 Schur right-hand side (m)        14 Schur CG solution before unscaling (m)
  *  15 P*Hg*P^T rowptr (n+1)  16 its colidx (nnz_hgam)  17 its value map (nnz_hgam)

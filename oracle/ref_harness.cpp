@@ -1,12 +1,11 @@
 // ref_harness.cpp -- extern "C" entry points over the reference's own host
-// sources (src/input_functions.cpp, src/MMatrix.cpp, src/permcheck.cu), which
-// oracle/Makefile compiles where they lie into oracle/_ref/libhykkt_ref.so.
-// TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle's ingest and
-// permutation-map restatements against the reference code itself.
+// sources (src/input_functions.cpp, src/MMatrix.cpp), which oracle/Makefile
+// compiles where they lie into oracle/_ref/libhykkt_ref.so.
+// TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle's and the
+// product's MatrixMarket ingest against the reference code itself.
 #include <cstring>
 #include "MMatrix.hpp"
 #include "input_functions.hpp"
-#include "permcheck.hpp"
 
 extern "C" {
 
@@ -30,19 +29,4 @@ int ref_read_csr(const char* path, int skip, int symmetric, int* n, int* m, int 
 // read_rhs (input_functions.cpp:198-217).
 void ref_read_rhs(const char* path, double* out) { read_rhs(path, out); }
 
-// permcheck.cu:99-195
-void ref_reverse_perm(int n, int* perm, int* rev) { reverse_perm(n, perm, rev); }
-void ref_map_rc(int n, int* rows, int* cols, int* perm, int* rev, int* prows, int* pcols,
-                int* pmap)
-{
-  make_vec_map_rc(n, rows, cols, perm, rev, prows, pcols, pmap);
-}
-void ref_map_c(int n, int* rows, int* cols, int* rev, int* pcols, int* pmap)
-{
-  make_vec_map_c(n, rows, cols, rev, pcols, pmap);
-}
-void ref_map_r(int n, int* rows, int* cols, int* perm, int* prows, int* pcols, int* pmap)
-{
-  make_vec_map_r(n, rows, cols, perm, prows, pcols, pmap);
-}
 }

@@ -331,16 +331,12 @@ def kkt_residual(H, Ds_vals, Jc, Jd, rx, rs, ry, ryd, dx, dsv, dyc, dyd):
 
 # ------------------------------------------------------- reference host code
 def ref_lib():
-    """The reference's own input_functions/MMatrix/permcheck host code, or None."""
+    """The reference's own input_functions/MMatrix host code, or None."""
     global _ref
     if _ref is None and os.path.exists(REF_SO):
         R = C.CDLL(REF_SO)
         R.ref_read_csr.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.c_int, _i, _i, _d]
         R.ref_read_rhs.argtypes = [C.c_char_p, _d]
-        R.ref_reverse_perm.argtypes = [C.c_int, _i, _i]
-        R.ref_map_rc.argtypes = [C.c_int, _i, _i, _i, _i, _i, _i, _i]
-        R.ref_map_c.argtypes = [C.c_int, _i, _i, _i, _i, _i]
-        R.ref_map_r.argtypes = [C.c_int, _i, _i, _i, _i, _i, _i]
         _ref = R
     return _ref

@@ -24,3 +24,28 @@ def load(mats, kind, step):
     vec = lambda k: O.read_rhs(mats(fl[k]))
     return dict(H=H, Ds=Ds, Jc=Jc, Jd=Jd, rx=vec("rx"), rs=vec("rs"), ry=vec("ry"),
                 ryd=vec("ryd"), skip=sk, files=fl)
+
+
+def hgamma_in_factor_order(solver, nnz_hgam, n, hgam_natural):
+    """The oracle's H_gamma values (natural CSR order) rearranged the way
+    debug selector 10 returns what the factorization received: P*Hg*P^T CSR
+    order (selectors 15/16, value map 17), NaN above the diagonal."""
+    import numpy as np
+    prp = solver.debug(15, n + 1, np.int32)
+    pci = solver.debug(16, nnz_hgam, np.int32)
+    pmap = solver.debug(17, nnz_hgam, np.int32)
+    rows = np.repeat(np.arange(n), np.diff(prp))
+    want = hgam_natural[pmap].copy()
+    want[pci > rows] = np.nan
+    return want
+
+
+def same_entries(got, want):
+    """Same NaN positions, and the other entries equal bit for bit."""
+    import numpy as np
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    if got.shape != want.shape:
+        return False
+    nan = np.isnan(want)
+    return (np.array_equal(np.isnan(got), nan)
+            and np.array_equal(got[~nan].view(np.int64), want[~nan].view(np.int64)))

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+import fixtures as F
 import hykkt_amd as H
 from hykkt_amd.generator import ds_spread_schedule, workload
 
@@ -57,7 +58,9 @@ def test_grid10k_step_vs_oracle(seq, step):
     assert st["nnz_hgam"] == ref["stats"]["nnz_hgam"]
     assert np.array_equal(s.debug(11, st["nnz_htil"]), ref["htil_v"])
     assert np.array_equal(s.debug(4, st["n"] + st["m"]), ref["max_d"])
-    assert np.array_equal(s.debug(10, st["nnz_hgam"]), ref["hgam_v"])
+    # H_gamma as the factorization received it, read back from the factor storage
+    want = F.hgamma_in_factor_order(s, st["nnz_hgam"], st["n"], ref["hgam_v"])
+    assert F.same_entries(s.debug(10, st["nnz_hgam"]), want)
     assert np.array_equal(s.debug(12, st["n"]), ref["rx_hat"])
     mine = s.solution()
     x = np.concatenate(mine)

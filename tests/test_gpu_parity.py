@@ -196,7 +196,10 @@ def test_execute_fixture_vs_oracle(mats, kind, step):
     # bit-exact formation: H~ (before Ruiz), max_d, H_gamma (after Ruiz)
     assert np.array_equal(solver.debug(11, st["nnz_htil"]), ref["htil_v"])
     assert np.array_equal(solver.debug(4, st["n"] + st["m"]), ref["max_d"])
-    assert np.array_equal(solver.debug(10, st["nnz_hgam"]), ref["hgam_v"])
+    # H_gamma as the factorization received it, read back from the factor storage
+    got = solver.debug(10, st["nnz_hgam"])
+    want = F.hgamma_in_factor_order(solver, st["nnz_hgam"], st["n"], ref["hgam_v"])
+    assert F.same_entries(got, want)
     assert np.array_equal(solver.debug(12, st["n"]), ref["rx_hat"])
     x = np.concatenate(mine)
     xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
@@ -240,6 +243,12 @@ def test_read_matrix_files_path(mats):
                         fl["skip"])
     assert s.execute() == 0
     assert s.stats()["resid4"] <= 1e-8
+    # the solution of the file path against the oracle's solve of the same files
+    sy = F.load(mats, F.A200, "10")
+    ref = O.execute(sy["H"], sy["Ds"].v, sy["Jc"], sy["Jd"], sy["rx"], sy["rs"], sy["ry"],
+                    sy["ryd"], 1e4, perm=s.permutation())
+    xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
+    assert rel(np.concatenate(s.solution()), xo) < 1e-6
     # a different Jd emptiness on the file path is refused (HykktSolver.cpp:140-144,243-250)
     fl0 = F.files(F.A200, "00")
     s.read_matrix_files(*[mats(fl0[k]) for k in ("H", "Ds", "Jc", "Jd", "rx", "rs", "ry", "ryd")],

@@ -137,3 +137,54 @@ def test_errors_are_codes_not_exits():
     rp = np.array([0, 1], np.int32)
     r = H.lib().hykkt_amd_order(1, rp.ctypes.data, None, None)
     assert r == -1 and b"null" in H.lib().hykkt_last_error()
+
+
+def _fixture_files():
+    """Every fixture file of the reference (src/mats) with the skip count its
+    CTest case or driver uses and whether it is read as a symmetric block."""
+    out = []
+    base = os.path.join(ROOT, "tests", "golden", "mats")
+    for d in sorted(os.listdir(base)):
+        skip = 2 if d == "a200" else 3
+        for f in sorted(os.listdir(os.path.join(base, d))):
+            rel = f"{d}/{f[:-3]}"
+            if "rhs" in f or f.startswith(("block_rx", "block_rs", "block_ry")):
+                out.append((rel, 0, None))
+            else:
+                out.append((rel, skip, False))
+                if f.startswith("block_H_matrix"):   # read_matrix_files: H is symmetric
+                    out.append((rel, skip, True))
+    return out
+
+
+@pytest.mark.parametrize("rel,skip,sym", _fixture_files())
+def test_product_ingest_matches_oracle_and_reference(mats, rel, skip, sym):
+    """hykkt_mm_read_csr / hykkt_mm_read_rhs (the product's readers behind
+    read_matrix_files and the drivers) against the oracle and, where it is
+    built, the reference's own input_functions.cpp, bit for bit."""
+    import ctypes as C
+    path = mats(rel)
+    R = O.ref_lib()
+    if sym is None:
+        mine = H.mm_read_rhs(path)
+        ref = O.read_rhs(path)
+        assert np.array_equal(mine, ref) and len(mine) > 0
+        if R is not None:
+            y = np.zeros(len(ref))
+            R.ref_read_rhs(path.encode(), y)
+            assert np.array_equal(mine, y)
+        return
+    n, m, rp, ci, v = H.mm_read_csr(path, skip, sym)
+    A = O.read_mm(path, skip, sym)
+    assert (n, m) == (A.n, A.m)
+    assert np.array_equal(rp, A.rp) and np.array_equal(ci, A.ci) and np.array_equal(v, A.v)
+    if R is not None:
+        nn, mm = C.c_int(), C.c_int()
+        rrp = np.zeros(n + 2, np.int32)
+        rci = np.zeros(len(ci) + 1, np.int32)
+        rcv = np.zeros(len(ci) + 1)
+        k = R.ref_read_csr(path.encode(), skip, int(sym), C.byref(nn), C.byref(mm), len(rci),
+                           rrp, rci, rcv)
+        assert k == len(ci) and (nn.value, mm.value) == (n, m)
+        assert np.array_equal(rrp[:n + 1], rp)
+        assert np.array_equal(rci[:k], ci) and np.array_equal(rcv[:k], v)

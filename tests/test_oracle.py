@@ -237,38 +237,6 @@ def test_read_rhs_reference(mats):
     assert np.array_equal(x, y)
 
 
-def test_perm_maps_reference_sources():
-    """Oracle maps == the reference's permcheck.cu host functions, on a random
-    symmetric pattern and a rectangular one."""
-    R = O.ref_lib()
-    if R is None:
-        pytest.skip("oracle/_ref not built")
-    A, _ = rand_csr(300, 300, 0.02, RNG, sym=True, diag=1.0)
-    perm = RNG.permutation(300).astype(np.int32)
-    rev, mp = O.perm_maps(A.rp, A.ci, perm)
-    rrev = np.zeros(300, np.int32)
-    R.ref_reverse_perm(300, perm.copy(), rrev)
-    assert np.array_equal(rev, rrev)
-    prp = np.zeros(301, np.int32)
-    pci = np.zeros(A.nnz, np.int32)
-    pm = np.zeros(A.nnz, np.int32)
-    R.ref_map_rc(300, A.rp.copy(), A.ci.copy(), perm.copy(), rev.copy(), prp, pci, pm)
-    assert all(np.array_equal(a, b) for a, b in zip(mp["rc"], (prp, pci, pm)))
-    J, _ = rand_csr(120, 300, 0.03, RNG)
-    _, mpj = O.perm_maps(J.rp, J.ci, perm, nrows=120)
-    pci = np.zeros(J.nnz, np.int32)
-    pm = np.zeros(J.nnz, np.int32)
-    R.ref_map_c(120, J.rp.copy(), J.ci.copy(), rev.copy(), pci, pm)
-    assert np.array_equal(mpj["c"][0], pci) and np.array_equal(mpj["c"][1], pm)
-    Jt = O.transpose(J)
-    _, mpt = O.perm_maps(Jt.rp, Jt.ci, perm)
-    prp = np.zeros(301, np.int32)
-    pci = np.zeros(J.nnz, np.int32)
-    pm = np.zeros(J.nnz, np.int32)
-    R.ref_map_r(300, Jt.rp.copy(), Jt.ci.copy(), perm.copy(), prp, pci, pm)
-    assert all(np.array_equal(a, b) for a, b in zip(mpt["r"], (prp, pci, pm)))
-
-
 # ---------------------------------------------------------------- kernels
 def test_transpose_and_spmv_dense():
     A, D = rand_csr(37, 53, 0.1, RNG)
@@ -333,6 +301,51 @@ def test_execute_matches_dense_kkt_lu():
     assert np.linalg.norm(mine - sol) / np.linalg.norm(sol) < 1e-8
     assert out["stats"]["resid4"] < 1e-8
     assert np.linalg.norm(K @ mine - rhs) / np.linalg.norm(rhs) < 1e-8
+
+
+@pytest.mark.parametrize("with_jd", [True, False])
+def test_kkt_residual_vs_dense(with_jd):
+    """orc_kkt_residual (HykktSolver.cpp:846-930 plus block rows 2 and 4)
+    against numpy's ||K x - rhs|| on the dense 4x4 block KKT matrix.  x is
+    the oracle's solution perturbed in one block at a time, so every block
+    row carries a residual far above rounding; a dropped row, a wrong sign
+    or a wrong denominator changes a value by O(1).  resid2 is rows 1 and 3
+    (the reference's form), resid4 all four rows, both over ||rhs||."""
+    g = GridPattern(20, 26, seed=4, gamma=1e4, ds_range=(1e-3, 1e3))
+    s = g.system()
+    c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
+    H, Jc, Jd = c(s.H), c(s.Jc), c(s.Jd)
+    if not with_jd:
+        Jd = O.Csr(Jd.n, Jd.m, np.zeros(Jd.n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    n, m, d = H.n, Jc.n, len(s.Ds.v)
+    out = O.execute(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd, s.gamma)
+    K = dense_kkt(H, s.Ds.v, Jc, Jd)
+    rhs = np.concatenate([s.rx, s.rs, s.ry, s.ryd])
+    x0 = np.concatenate([out["dx"], out["ds"], out["dyc"], out["dyd"]])
+    off = np.cumsum([0, n, d, m, d])
+    rng = np.random.default_rng(1)
+    for blk in range(4):
+        x = x0.copy()
+        x[off[blk]:off[blk + 1]] += 1e-3 * rng.standard_normal(off[blk + 1] - off[blk])
+        res = K @ x - rhs
+        rows = [res[off[k]:off[k + 1]] for k in range(4)]
+        nb = np.linalg.norm(rhs)
+        want4 = np.linalg.norm(res) / nb
+        want2 = np.sqrt(rows[0] @ rows[0] + rows[2] @ rows[2]) / nb
+        got2, got4 = O.kkt_residual(H, s.Ds.v, Jc, Jd, s.rx, s.rs, s.ry, s.ryd,
+                                    *(x[off[k]:off[k + 1]] for k in range(4)))
+        # rows the perturbation does not reach keep the solution's own
+        # residual (~1e-10 here), which the two evaluation orders round
+        # differently: 1e-13 absolute covers that, far below the O(1e-3)
+        # values of the perturbed rows
+        assert want4 > 1e-6, blk   # the perturbation dominates rounding
+        assert abs(got4 - want4) <= 1e-12 * want4 + 1e-13, (blk, got4, want4)
+        assert abs(got2 - want2) <= 1e-12 * want2 + 1e-13, (blk, got2, want2)
+        # which rows the perturbed block reaches (column structure of K)
+        touched = [np.linalg.norm(r) > 1e-8 for r in rows]
+        expect = {0: [True, False, True, with_jd], 1: [False, True, False, True],
+                  2: [True, False, False, False], 3: [with_jd, True, False, False]}[blk]
+        assert touched == expect, (blk, touched)
 
 
 @pytest.mark.parametrize("kind,step,nnz_htil,nnz_hgam", [

@@ -126,22 +126,25 @@ def cpu_baseline(g, perm, nsample, gpu_solutions):
             "max_resid4": worst, "gpu_err_vs_oracle": errs}
 
 
-def served_vs_plain(rate, seconds, npat=16, budget=200e9):
+def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0):
     """The served form (continuous batching over 16 resident patterns of 1k,
-    5k and 20k buses, 6:3:1) and the plain path (one request at a time on the
-    same resident solvers) on the same Poisson request stream.  Both are over
-    capacity at 2,000 requests/s: the figures are saturation throughput and
-    queueing latency over `seconds`."""
+    5k and 20k buses, 6:3:1) and the plain path (one request at a time through
+    the drop-in solver of each pattern) on the same Poisson request stream.
+    Both are over capacity at 2,000 requests/s: the figures are saturation
+    throughput and queueing latency over `seconds`, p50/p99 over the requests
+    that arrived after `warmup` seconds."""
     from hykkt_amd.served import Server, make_patterns, poisson_requests, summarize
     t0 = time.perf_counter()
     pats = make_patterns(npat, seed=100)
     srv = Server(pats, hbm_budget=budget)
     setup = time.perf_counter() - t0
+    print(f"bench: served setup {setup:.1f} s, {len(srv.served)} patterns, "
+          f"{srv.hbm_used / 1e9:.1f} GB, capacities {srv.capacity}", file=sys.stderr, flush=True)
     reqs = poisson_requests(pats, rate, seconds, seed=7, served=srv.served)
     res = {}
     for mode in ("served", "plain"):
         res[mode] = summarize(*srv.run(reqs, plain=(mode == "plain"), deadline=seconds),
-                              patterns=pats, solo_s=srv.solo_s)
+                              patterns=pats, solo_s=srv.solo_s, warmup=warmup)
         print(f"bench: {mode} stream: {res[mode]}", file=sys.stderr, flush=True)
     srv.close()
     for p in pats:
@@ -149,8 +152,12 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9):
     sp, pp = res["served"]["solves_per_s"], res["plain"]["solves_per_s"]
     sw, pw = res["served"]["solo_s_per_s"], res["plain"]["solo_s_per_s"]
     return {"stream": f"Poisson {rate:.0f} requests/s for {seconds:.0f} s over {npat} resident "
-                      "patterns (1k/5k/20k buses, 6:3:1), fresh values and rhs per request, "
+                      "patterns (1k/5k/20k buses, 6:3:1); a pattern's i-th request is its "
+                      f"interior-point step i, values from a pool of {len(pats[0].pool)} fresh "
+                      "perturbations per pattern with the Ds spread growing with the step, "
                       "gamma 1e5; requests past the window stay pending",
+            "capacity_by_pattern": {str(pats[p].nbus) + "/" + str(p): c
+                                    for p, c in srv.capacity.items()},
             "patterns_served": len(srv.served), "hbm_used_gb": srv.hbm_used / 1e9,
             "setup_s": setup, "served": res["served"], "plain": res["plain"],
             "solo_ms": {str(pats[p].nbus) + "/" + str(p): 1e3 * t for p, t in srv.solo_s.items()},
@@ -160,13 +167,13 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9):
 
 def batched_vs_plain():
     """The batched form (256 systems: 32 patterns of 200..5000 buses x 8 value
-    draws, a gamma per system, solved concurrently on one resident slot per
-    pattern) against the same systems one at a time on the same slots."""
-    from hykkt_amd.batched import make_batch, run_batch, summarize_batch
-    from hykkt_amd.served import Server
+    draws, a gamma per system; each pattern's 8 draws in one pass of the batch
+    engine, the patterns' passes concurrent) against the same systems one at a
+    time through the drop-in path."""
+    from hykkt_amd.batched import make_batch, make_server, run_batch, summarize_batch
     t0 = time.perf_counter()
     pats = make_batch(seed=3)
-    srv = Server(pats, hbm_budget=200e9)
+    srv = make_server(pats, hbm_budget=200e9)
     setup = time.perf_counter() - t0
     if len(srv.served) != len(pats):
         raise RuntimeError(f"batched: slots refused: {srv.refused}")   # over the budget
@@ -198,7 +205,7 @@ def main():
     ap.add_argument("--kernel-timing", type=int, default=1)
     ap.add_argument("--batched", type=int, default=1,
                     help="run the 256-system batch, batched vs one at a time")
-    ap.add_argument("--served-seconds", type=float, default=6.0,
+    ap.add_argument("--served-seconds", type=float, default=60.0,
                     help="length of the served and plain request streams (0: skip)")
     ap.add_argument("--served-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served stream (requests/s)")

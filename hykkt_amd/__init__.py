@@ -94,6 +94,13 @@ _SIGS = {
     "hykkt_get_stats": ([_vp, C.POINTER(Stats)], C.c_int),
     "hykkt_get_solution": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_get_solution_device": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
+    "hykkt_create_batch": ([C.c_double, C.c_int, C.POINTER(_vp)], C.c_int),
+    "hykkt_batch_capacity": ([_vp, _ip], C.c_int),
+    "hykkt_bind_system": ([_vp, C.c_int, C.c_double] + [_vp] * 9, C.c_int),
+    "hykkt_execute_batch": ([_vp, C.c_int, _ip], C.c_int),
+    "hykkt_get_stats_k": ([_vp, C.c_int, C.POINTER(Stats)], C.c_int),
+    "hykkt_get_solution_k": ([_vp, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
+    "hykkt_get_solution_device_k": ([_vp, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_get_permutation": ([_vp, _vp, C.c_int], C.c_int),
     "hykkt_set_permutation": ([_vp, _vp, C.c_int], C.c_int),
     "hykkt_debug_copy": ([_vp, C.c_int, _vp, C.c_longlong], C.c_longlong),
@@ -314,6 +321,51 @@ class HykktSolver:
         if r < 0:
             raise HykktError(int(r), lib().hykkt_last_error().decode())
         return out[:r]
+
+
+class BatchSolver(HykktSolver):
+    """Batched form (no counterpart in the reference): up to `capacity`
+    systems sharing one sparsity pattern, solved together, each with exactly
+    the arithmetic it gets alone.  The pattern comes from the set_H / set_Ds /
+    set_Jc / set_Jd setters (their values are not used); system k's values
+    from bind_system(k, ...)."""
+
+    def __init__(self, gamma: float, capacity: int):
+        self._h = C.c_void_p()
+        _check(lib().hykkt_create_batch(float(gamma), int(capacity), C.byref(self._h)))
+        self._keep = {}
+        self.capacity = int(capacity)
+
+    def bind_system(self, k, gamma, H, Ds, Jc, Jd, rx, rs, ry, ryd, y0=None):
+        """Value arrays of system k (numpy arrays, or DeviceArray / device
+        addresses with inputs_on_device)."""
+        arrs = [H, Ds, Jc, Jd, rx, rs, ry, ryd, y0]
+        if not self.options().inputs_on_device:
+            arrs = [None if a is None else _f64(a) for a in arrs]
+        self._keep[("sys", int(k))] = arrs
+        _check(lib().hykkt_bind_system(self._h, int(k), float(gamma), *[_p(a) for a in arrs]))
+
+    def execute_batch(self, count) -> list:
+        st = np.zeros(int(count), np.int32)
+        _check(lib().hykkt_execute_batch(self._h, int(count), _p(st)))
+        return [int(v) for v in st]
+
+    def stats_k(self, k) -> dict:
+        st = Stats()
+        _check(lib().hykkt_get_stats_k(self._h, int(k), C.byref(st)))
+        return st.as_dict()
+
+    def solution_k(self, k):
+        st = self.stats_k(k)
+        n, m, d = st["n"], st["m"], st["d"]
+        dx, ds, dyc, dyd = np.zeros(n), np.zeros(d), np.zeros(m), np.zeros(d)
+        _check(lib().hykkt_get_solution_k(self._h, int(k), _p(dx), _p(ds), _p(dyc), _p(dyd)))
+        return dx, ds, dyc, dyd
+
+    def solution_device_k(self, k):
+        ptrs = [C.c_void_p() for _ in range(4)]
+        _check(lib().hykkt_get_solution_device_k(self._h, int(k), *[C.byref(p) for p in ptrs]))
+        return [p.value for p in ptrs]
 
 
 class DeviceArray:

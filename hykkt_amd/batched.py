@@ -2,18 +2,17 @@
 
 A batch is a set of systems over a few distinct sparsity patterns (several
 value draws per pattern, each with its own gamma).  Every pattern gets one
-resident solver slot (symbolic analysis, permutation and factor storage kept
-on the device; admission by HBM budget, as in the served form), and the
-batch's systems run through the slots concurrently: one host worker per slot
-takes its pattern's systems in turn, and the slots' kernels overlap on the
-device.  The same systems run one at a time on the same slots are the plain
-comparison; each result is bit-identical either way.  A system whose H_gamma
-is not positive definite (the generator's random H can make H~ indefinite on
-the null space of Jc) fails at a non-positive pivot, as in the reference, and
-is counted as such.
-
-This is batching by concurrency of per-pattern solves.  It does not fuse
-several systems into one kernel launch.
+resident batch slot (symbolic analysis, permutation, and per-system factor
+storage and workspace for all its draws; admission by HBM budget, as in the
+served form).  All the draws of a pattern are solved in one pass through the
+batch engine -- formation, Ruiz, the numeric Cholesky, the triangular sweeps
+and the device-resident Schur CG each launch once for all of them -- and the
+patterns' passes run concurrently on their own streams, so systems of
+different sizes overlap on the device.  The same systems run one at a time
+through the drop-in path are the plain comparison; each result is
+bit-identical either way.  A system whose H_gamma is not positive definite
+(the generator's random H can make H~ indefinite on the null space of Jc)
+fails at a non-positive pivot, as in the reference, and is counted as such.
 """
 from __future__ import annotations
 
@@ -51,6 +50,12 @@ def batch_requests(patterns):
             reqs.append(Request(rid, p.pid, k, 0.0))
             rid += 1
     return reqs
+
+
+def make_server(patterns, hbm_budget: float = 200e9, keep_solutions: bool = False):
+    """One batch slot per pattern holding all its draws."""
+    draws = max(len(p.pool) for p in patterns)
+    return Server(patterns, hbm_budget, capacity=draws, keep_solutions=keep_solutions)
 
 
 def run_batch(server: Server, patterns, plain: bool = False):

@@ -14,7 +14,7 @@
 
 struct hykkt_solver_s {
   hykkt::Solver impl;
-  explicit hykkt_solver_s(double g) : impl(g) {}
+  explicit hykkt_solver_s(double g, int cap = 1) : impl(g, cap) {}
 };
 
 struct hykkt_chol_s {
@@ -72,7 +72,7 @@ int hykkt_default_options(hykkt_options* o)
     o->cg_itmax = 100;
     o->delta = 0.0;
     o->inputs_on_device = 0;
-    o->cg_check_every = 1;
+    o->cg_check_every = 0;
     o->verbose = 0;
     o->refine_max = 3;
     o->refine_tol = 9e-9;   // the 1e-8 accuracy target less a 10% margin
@@ -203,9 +203,7 @@ int hykkt_set_options(hykkt_solver s, const hykkt_options* o)
     HYKKT_REQUIRE(o->ruiz_its >= 0 && o->cg_itmax >= 0 && o->cg_check_every >= 0 &&
                       o->refine_max >= 0,
                   HYKKT_E_ARG, "negative iteration count");
-    bool dev_changed = o->inputs_on_device != s->impl.opts.inputs_on_device;
     s->impl.opts = *o;
-    (void)dev_changed;
     return HYKKT_OK;
   });
 }
@@ -244,7 +242,81 @@ int hykkt_get_solution_device(hykkt_solver s, const double** dx, const double** 
 {
   return guard([&] {
     NEED(s);
-    s->impl.get_solution_device(dx, ds, dyc, dyd);
+    s->impl.get_solution_device(0, dx, ds, dyc, dyd);
+    return HYKKT_OK;
+  });
+}
+
+// ------------------------------------------------------------------ batched form
+int hykkt_create_batch(double gamma, int capacity, hykkt_solver* out)
+{
+  return guard([&] {
+    NEED(out);
+    HYKKT_REQUIRE(capacity >= 1, HYKKT_E_ARG, "batch capacity must be at least 1");
+    *out = new hykkt_solver_s(gamma, capacity);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_batch_capacity(hykkt_solver s, int* capacity)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(capacity);
+    *capacity = s->impl.capacity();
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_bind_system(hykkt_solver s, int k, double gamma, const double* h_vals,
+                      const double* ds_vals, const double* jc_vals, const double* jd_vals,
+                      const double* rx, const double* rs, const double* ry, const double* ryd,
+                      const double* y0)
+{
+  return guard([&] {
+    NEED(s);
+    const double* v[hykkt::IN_COUNT] = {h_vals, ds_vals, jc_vals, jd_vals, rx, rs, ry, ryd, y0};
+    s->impl.bind_system(k, gamma, v);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_execute_batch(hykkt_solver s, int count, int* status)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(status);
+    const std::vector<int> rc = s->impl.execute_batch(count);
+    std::copy(rc.begin(), rc.end(), status);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_stats_k(hykkt_solver s, int k, hykkt_stats* st)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(st);
+    *st = s->impl.stats_of(k);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_solution_k(hykkt_solver s, int k, double* dx, double* ds, double* dyc, double* dyd)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.get_solution(k, dx, ds, dyc, dyd);
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_solution_device_k(hykkt_solver s, int k, const double** dx, const double** ds,
+                                const double** dyc, const double** dyd)
+{
+  return guard([&] {
+    NEED(s);
+    s->impl.get_solution_device(k, dx, ds, dyc, dyd);
     return HYKKT_OK;
   });
 }
@@ -345,7 +417,7 @@ int hykkt_device_copy_f64(double* dst, const double* src, long long count)
     if (count) {
       NEED(dst);
       NEED(src);
-      hykkt::dev::copy(dst, src, count, nullptr);
+      hykkt::dev::copy(dst, src, count, hykkt::dev::Batch{}, nullptr);
     }
     return HYKKT_OK;
   });
@@ -421,11 +493,13 @@ int hykkt_ruiz_scale(int its, int n, int m2, const int* h_rp, const int* h_ci, d
     md.alloc(std::max(1, n + m2));
     sc.alloc(std::max(1, n + m2));
     hipStream_t st = nullptr;
-    if (its == 0) hykkt::dev::fill(md.p, 1.0, n + m2, st);
+    const hykkt::dev::Batch one;
+    if (its == 0) hykkt::dev::fill(md.p, 1.0, n + m2, one, st);
     for (int it = 0; it < its; it++) {
-      hykkt::dev::ruiz_rowmax(n, n + m2, hrp.p, hv.p, jtrp.p, nullptr, jtv.p, jrp.p, jv.p, sc.p, st);
+      hykkt::dev::ruiz_rowmax(n, n + m2, hrp.p, hv.p, jtrp.p, nullptr, jtv.p, jrp.p, jv.p, sc.p,
+                              nullptr, one, st);
       hykkt::dev::ruiz_scale(n, n + m2, hrp.p, hci.p, hv.p, jtrp.p, jtci.p, jtv.p, jrp.p, jci.p,
-                             jv.p, sc.p, r1.p, r2.p, md.p, it == 0, st);
+                             jv.p, sc.p, r1.p, r2.p, md.p, it == 0, one, st);
     }
     HIPCHK(hipDeviceSynchronize());
     auto down = [](double* dst, const DevBuf<double>& b, int cnt) {
@@ -550,6 +624,7 @@ int hykkt_chol_create(int n, int nnz, const int* rp, const int* ci, hykkt_chol* 
     HIPCHK(hipStreamCreate(&c->st));
     hykkt::Pattern a = make_pattern(n, n, rp, ci);
     c->eng = std::make_unique<hykkt::CholEngine>(a, hykkt::CholOptions{}, c->st);
+    c->eng->own(1);
     c->vals.alloc(std::max(1, nnz));
     c->x.alloc(std::max(1, n));
     c->post = c->eng->plan().post;
@@ -567,8 +642,9 @@ int hykkt_chol_factor(hykkt_chol c, const double* vals, double tol, int* singula
     NEED(vals);
     if (c->nnz)
       HIPCHK(hipMemcpyAsync(c->vals.p, vals, c->nnz * sizeof(double), hipMemcpyHostToDevice, c->st));
-    c->eng->assemble(c->vals.p);
-    c->eng->factor(tol);
+    const hykkt::dev::Batch one{1, c->eng->stride(), nullptr};
+    c->eng->assemble(c->vals.p, one);
+    c->eng->factor(tol, one);
     int piv = c->eng->pivot();
     if (singularity) *singularity = piv;
     return piv >= 0 ? HYKKT_E_PIVOT : HYKKT_OK;
@@ -635,8 +711,23 @@ int hykkt_schur_cg(hykkt_chol c, int m, int n, const int* j_rp, const int* j_ci,
     std::vector<double> jhv(jmap.size()), jthv(jt.map.size());
     for (size_t q = 0; q < jmap.size(); q++) jhv[q] = j_v[jmap[q]];
     for (size_t q = 0; q < jt.map.size(); q++) jthv[q] = jhv[jt.map[q]];
-    DevBuf<int> jrp, jci, jtrp, jtci;
-    DevBuf<double> jv, jtv, t1, xv, r, w, p, s, bb, sc, part;
+    // row blocks of the LDS-staged SpMVs: whole rows, at most 256 of them and
+    // at most spmv_lds_entries() entries per block (a longer row alone)
+    auto blocks = [](const std::vector<int>& rp, int rows) {
+      const int E = hykkt::dev::spmv_lds_entries();
+      std::vector<int> bl{0};
+      for (int r = 0; r < rows;) {
+        int r1 = r + 1;
+        if (rp[r + 1] - rp[r] <= E)
+          while (r1 < rows && r1 - r < 256 && rp[r1 + 1] - rp[r] <= E) r1++;
+        bl.push_back(r1);
+        r = r1;
+      }
+      return bl;
+    };
+    const std::vector<int> jb = blocks(Jh.rp, m), jtb = blocks(jt.at.rp, n);
+    DevBuf<int> jrp, jci, jtrp, jtci, jblk, jtblk, flg;
+    DevBuf<double> jv, jtv, t1, xv, r, w, p, s, bb, sc, part, gath;
     auto upi = [](DevBuf<int>& d, const std::vector<int>& v) {
       if (v.empty()) d.upload(std::vector<int>{0});
       else d.upload(v);
@@ -651,20 +742,28 @@ int hykkt_schur_cg(hykkt_chol c, int m, int n, const int* j_rp, const int* j_ci,
     upi(jtrp, jt.at.rp);
     upi(jtci, jt.at.ci);
     upd(jtv, jthv);
+    upi(jblk, jb);
+    upi(jtblk, jtb);
     upd(xv, std::vector<double>(x, x + m));
     upd(bb, std::vector<double>(b, b + m));
     t1.alloc(std::max(1, n));
     for (DevBuf<double>* q : {&r, &w, &p, &s}) q->alloc(std::max(1, m));
     sc.alloc(8);
-    part.alloc(hykkt::dev::reduce_partials());
+    part.alloc(hykkt::schur_partials(m, static_cast<int>(jb.size()) - 1));
+    gath.alloc(2);
+    flg.alloc(3);
     hykkt::SchurOps o;
     o.m = m;
     o.n = n;
     o.jh_rp = jrp.p;
     o.jh_ci = jci.p;
+    o.jh_blk = jblk.p;
+    o.jh_nblk = static_cast<int>(jb.size()) - 1;
     o.jh_v = jv.p;
     o.jth_rp = jtrp.p;
     o.jth_ci = jtci.p;
+    o.jth_blk = jtblk.p;
+    o.jth_nblk = static_cast<int>(jtb.size()) - 1;
     o.jth_v = jtv.p;
     o.chol = c->eng.get();
     o.st = c->st;
@@ -676,14 +775,21 @@ int hykkt_schur_cg(hykkt_chol c, int m, int n, const int* j_rp, const int* j_ci,
     o.s = s.p;
     o.sc = sc.p;
     o.part = part.p;
-    int it = 0;
-    double fn = 0.0;
-    int rc = hykkt::schur_cg_run(o, bb.p, tol, itmax, 1, &it, &fn);
+    o.live = flg.p;
+    o.iters = flg.p + 1;
+    o.status = flg.p + 2;
+    o.gath = gath.p;
+    // one system: its vectors are separate buffers, the factor the engine's
+    const hykkt::dev::Batch one{1, c->eng->stride(), nullptr};
+    const std::vector<hykkt::CgResult> res =
+        hykkt::schur_cg_run(o, bb.p, tol, itmax, one, false, 1, 1);
     HIPCHK(hipMemcpyAsync(x, xv.p, m * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (iters) *iters = it;
-    if (final_norm) *final_norm = fn;
-    return rc;
+    HYKKT_REQUIRE(!c->eng->solve_error(), HYKKT_E_STATE,
+                  "triangular-solve sweep exceeded its wait bound (results invalid)");
+    if (iters) *iters = res[0].iters;
+    if (final_norm) *final_norm = res[0].final_norm;
+    return res[0].status;
   });
 }
 

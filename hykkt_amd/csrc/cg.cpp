@@ -1,11 +1,16 @@
-// cg.cpp -- Schur-complement conjugate gradient on the device.
+// cg.cpp -- Schur-complement conjugate gradient on the device, for a batch of
+// same-pattern systems.
 //
 // Same recurrences as SchurComplementConjugateGradient.cpp:102-194
 // (Chronopoulos-Gear form: one S-apply and two dots per iteration):
 //   p = beta p + r;  s = beta s + w;  x += alpha p;  r -= alpha s;
 //   gam1 = r.r;  stop if sqrt(gam1) < tol;  w = S r;  delta = w.r;
 //   beta = gam1/gam;  gam = gam1;  alpha = gam/(delta - beta gam/alpha).
-// Scalars stay on the device; only gam1 is read back for the stopping test.
+// Device-resident: the vector update carries the r.r partials, the second
+// SpMV of the S-apply carries the w.r partials, and the stopping test and
+// the scalar recurrences run in the one-workgroup final reductions, which
+// clear a system's live flag when it stops; every later kernel skips it.
+// The host only reads the live flags between chunks of iterations.
 #include <algorithm>
 #include <cmath>
 
@@ -15,62 +20,82 @@ namespace hykkt {
 
 namespace {
 
-// w = S v = Jh A^{-1} Jh^T v  (beta_out = 0) or out = alpha*S v + beta*y_in
+// w = S v = Jh A^{-1} Jh^T v, or out = alpha*S v + beta*y_in; with dot_w, the
+// partials of out.dot_w go to o.part
 void apply_s(SchurOps& o, const double* v, double alpha, double beta, const double* y_in,
-             double* out)
+             double* out, const dev::Batch& b, const double* dot_w = nullptr)
 {
-  dev::spmv(o.n, o.jth_rp, o.jth_ci, nullptr, o.jth_v, 1.0, v, 0.0, nullptr, o.t1, o.st);
-  o.chol->solve(o.t1);
-  dev::spmv(o.m, o.jh_rp, o.jh_ci, nullptr, o.jh_v, alpha, o.t1, beta, y_in, out, o.st);
+  dev::spmv(o.n, o.jth_rp, o.jth_ci, nullptr, o.jth_v, 1.0, nullptr, v, 0.0, nullptr, o.t1,
+            o.jth_blk, o.jth_nblk, b, o.st);
+  o.chol->solve(o.t1, b);
+  dev::spmv(o.m, o.jh_rp, o.jh_ci, nullptr, o.jh_v, alpha, nullptr, o.t1, beta, y_in, out,
+            o.jh_blk, o.jh_nblk, b, o.st, dot_w, dot_w ? o.part : nullptr);
 }
 
 }  // namespace
 
-int schur_cg_run(SchurOps& o, const double* b, double tol, int itmax, int check_every,
-                 int* iters, double* final_norm, bool x0_zero)
+int schur_partials(int m, int jh_nblk) { return std::max(dev::reduce_blocks(m), jh_nblk); }
+
+std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int itmax,
+                                   const dev::Batch& bt, bool x0_zero, int chunk, int chunk_next)
 {
-  const int m = o.m;
-  double* sc = o.sc;
+  const int m = o.m, cnt = bt.count;
+  const dev::Batch live = dev::masked(bt, o.live);
+  const dev::CgFlags fl{o.live, o.iters, o.status};
+  std::vector<CgResult> res(cnt);
+  // every system taking part starts live; the others stay out
+  if (bt.mask)
+    HIPCHK(hipMemcpyAsync(o.live, bt.mask, sizeof(int) * cnt, hipMemcpyDeviceToDevice, o.st));
+  else
+    dev::fill_i(o.live, 1, cnt, o.st);
+  dev::fill_i(o.iters, 0, cnt, o.st);
+  dev::fill_i(o.status, 0, cnt, o.st);
   // setup(): p = s = b (:89-100); r = b - S x0 (:104-122).  With x0 = 0 the
   // product is exactly zero and fma(-1, 0, b) = b, so r = b bit for bit and
   // the solve is skipped.
-  dev::copy(o.p, b, m, o.st);
-  dev::copy(o.s, b, m, o.st);
-  if (x0_zero) dev::copy(o.r, b, m, o.st);
-  else apply_s(o, o.x, -1.0, 1.0, b, o.r);
-  dev::dot(m, o.r, o.r, o.part, sc + 0, o.st);
-  double h = 0.0;
-  HIPCHK(hipMemcpyAsync(&h, sc + 0, sizeof(double), hipMemcpyDeviceToHost, o.st));
-  HIPCHK(hipStreamSynchronize(o.st));
-  double gam1 = h;
-  *iters = 0;
-  if (h == 0.0) {
-    // x0 already solves the system; the reference would divide 0/0 here.
-    *final_norm = 0.0;
-    return 0;
-  }
+  dev::copy(o.p, b, m, live, o.st);
+  dev::copy(o.s, b, m, live, o.st);
+  if (x0_zero) dev::copy(o.r, b, m, live, o.st);
+  else apply_s(o, o.x, -1.0, 1.0, b, o.r, live);
+  // gam = r.r; r.r == 0: x0 already solves the system (the reference would
+  // divide 0/0 here) and the system stops with 0 iterations
+  dev::dot_partial(m, o.r, o.r, o.part, live, o.st);
+  dev::dot_final(dev::reduce_blocks(m), o.part, o.sc, 0, true, false, -1, tol, itmax, fl, live,
+                 o.st);
   // w = S r, delta = w.r, alpha = gam/delta (:124-144)
-  apply_s(o, o.r, 1.0, 0.0, nullptr, o.w);
-  dev::dot(m, o.w, o.r, o.part, sc + 2, o.st);
-  dev::cg_scalars(sc, 1, o.st);
-  const int every = std::max(1, check_every);
-  int i = 0;
-  for (; i < itmax; i++) {
-    dev::cg_update(m, sc, o.p, o.s, o.x, o.r, o.w, o.st);
-    dev::dot(m, o.r, o.r, o.part, sc + 1, o.st);
-    if ((i + 1) % every == 0 || i + 1 == itmax) {
-      HIPCHK(hipMemcpyAsync(&h, sc + 1, sizeof(double), hipMemcpyDeviceToHost, o.st));
+  apply_s(o, o.r, 1.0, 0.0, nullptr, o.w, live, o.r);
+  dev::dot_final(o.jh_nblk, o.part, o.sc, 2, false, true, -1, tol, itmax, fl, live, o.st);
+  std::vector<int> lv(cnt);
+  int next_check = std::max(1, chunk);
+  for (int i = 0; i < itmax; i++) {
+    dev::cg_update_dot(m, o.sc, o.p, o.s, o.x, o.r, o.w, o.part, live, o.st);
+    dev::dot_final(dev::reduce_blocks(m), o.part, o.sc, 1, true, false, i, tol, itmax, fl, live,
+                   o.st);
+    apply_s(o, o.r, 1.0, 0.0, nullptr, o.w, live, o.r);
+    dev::dot_final(o.jh_nblk, o.part, o.sc, 2, false, true, i, tol, itmax, fl, live, o.st);
+    if (i + 1 == next_check && i + 1 < itmax) {
+      HIPCHK(hipMemcpyAsync(lv.data(), o.live, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
       HIPCHK(hipStreamSynchronize(o.st));
-      gam1 = h;
-      if (std::sqrt(gam1) < tol) break;
+      if (std::none_of(lv.begin(), lv.end(), [](int v) { return v != 0; })) break;
+      next_check += std::max(1, chunk_next);
     }
-    apply_s(o, o.r, 1.0, 0.0, nullptr, o.w);
-    dev::dot(m, o.w, o.r, o.part, sc + 2, o.st);
-    dev::cg_scalars(sc, 0, o.st);
   }
-  *iters = (i < itmax) ? i + 1 : itmax;
-  *final_norm = std::sqrt(gam1);
-  return (i == itmax) ? 1 : 0;
+  // per system: iterations, status and ||r|| at exit (gam1, or gam for a
+  // system that stopped at the start)
+  std::vector<int> it(cnt), stt(cnt);
+  std::vector<double> sc(2 * static_cast<size_t>(cnt));
+  dev::collect(o.gath, o.sc, 2, dev::Batch{cnt, bt.vs, nullptr}, o.st);
+  HIPCHK(hipMemcpyAsync(it.data(), o.iters, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
+  HIPCHK(hipMemcpyAsync(stt.data(), o.status, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
+  HIPCHK(hipMemcpyAsync(sc.data(), o.gath, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost,
+                        o.st));
+  HIPCHK(hipStreamSynchronize(o.st));
+  for (int k = 0; k < cnt; k++) {
+    res[k].iters = it[k];
+    res[k].status = stt[k];
+    res[k].final_norm = std::sqrt(it[k] == 0 ? sc[2 * k] : sc[2 * k + 1]);
+  }
+  return res;
 }
 
 }  // namespace hykkt

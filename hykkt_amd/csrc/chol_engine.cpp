@@ -77,12 +77,8 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   diag_pos_.upload(dpos);
   uvec_off_.upload(uvo);
   dinv_off_.upload(dvo);
-  L_.alloc(static_cast<size_t>(std::max<int64_t>(pl_.l_off[pl_.ns], 1)));
-  U_.alloc(static_cast<size_t>(std::max<int64_t>(pl_.u_off[pl_.ns], 1)));
-  ubuf_.alloc(static_cast<size_t>(std::max<int64_t>(uvo[pl_.ns], 1)));
-  dinv_.alloc(static_cast<size_t>(std::max<int64_t>(dvo[pl_.ns], 1)));
-  dev::fill(dinv_.p, 0.0, static_cast<int64_t>(dinv_.n), st_);
-  pivot_.alloc(1);
+  n_ubuf_ = uvo[pl_.ns];
+  n_dinv_ = dvo[pl_.ns];
   cd_.n = pl_.n;
   cd_.ns = pl_.ns;
   cd_.sn_first = sn_first_.p;
@@ -94,13 +90,55 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   cd_.child_ptr = child_ptr_.p;
   cd_.child_list = child_list_.p;
   cd_.sn_parent = sn_parent_.p;
-  cd_.L = L_.p;
-  cd_.U = U_.p;
-  cd_.pivot = pivot_.p;
-  cd_.dinv = dinv_.p;
   cd_.dinv_off = dinv_off_.p;
   build_tasks();
   build_solve();
+}
+
+CholEngine::Sizes CholEngine::sizes() const
+{
+  auto at_least1 = [](int64_t v) { return std::max<int64_t>(v, 1); };
+  return Sizes{at_least1(pl_.l_off[pl_.ns]), at_least1(pl_.u_off[pl_.ns]), at_least1(n_dinv_),
+               at_least1(pl_.n), at_least1(n_ubuf_), at_least1(n_bpart_)};
+}
+
+void CholEngine::bind(const Storage& s, int64_t vs, int cap)
+{
+  cd_.L = s.L;
+  cd_.U = s.U;
+  cd_.dinv = s.dinv;
+  cd_.pivot = s.pivot;
+  ybuf_ = s.y;
+  sd_.ubuf = s.ubuf;
+  sd_.bpart = s.bpart;
+  vs_ = vs;
+  cap_ = cap;
+  // only the b x b lower part of an inverted diagonal block is ever written
+  dev::fill(cd_.dinv, 0.0, n_dinv_, dev::Batch{cap, vs, nullptr}, st_);
+}
+
+void CholEngine::own(int cap)
+{
+  const Sizes z = sizes();
+  auto al = [](int64_t v) { return (v + 31) / 32 * 32; };   // 256-byte slots
+  const int64_t vs = al(z.L) + al(z.U) + al(z.dinv) + al(z.y) + al(z.ubuf) + al(z.bpart);
+  own_.alloc(static_cast<size_t>(vs) * cap);
+  own_piv_.alloc(static_cast<size_t>(cap));
+  double* p = own_.p;
+  Storage s{};
+  s.L = p;
+  p += al(z.L);
+  s.U = p;
+  p += al(z.U);
+  s.dinv = p;
+  p += al(z.dinv);
+  s.y = p;
+  p += al(z.y);
+  s.ubuf = p;
+  p += al(z.ubuf);
+  s.bpart = p;
+  s.pivot = own_piv_.p;
+  bind(s, vs, cap);
 }
 
 CholEngine::~CholEngine()
@@ -411,8 +449,6 @@ void CholEngine::build_solve()
   upi(pull_src_, src);
   front_off_.upload(foff);
   bpart_off_.upload(bpo);
-  bpart_.alloc(static_cast<size_t>(std::max<int64_t>(bpo[ns], 1)));
-  ybuf_.alloc(static_cast<size_t>(std::max(pl_.n, 1)));
   flags_.alloc(5);   // wait-bound error word, four sweep item counters
   sd_.fwd_items = fwd_items_.p;
   sd_.bwd_items = bwd_items_.p;
@@ -422,56 +458,51 @@ void CholEngine::build_solve()
   sd_.pull_ptr = pull_ptr_.p;
   sd_.pull_src = pull_src_.p;
   sd_.front_off = front_off_.p;
-  sd_.ubuf = ubuf_.p;
-  sd_.bpart = bpart_.p;
   sd_.bpart_off = bpart_off_.p;
   sd_.err = flags_.p;
   sd_.ticket = flags_.p + 1;
   sd_.grid_fwd = n_cu_ * dev::forest_blocks_per_cu();
-  sd_.trace = nullptr;
   n_bpart_ = bpo[ns];
-  if (const char* tp = std::getenv("HYKKT_SOLVE_TRACE")) {
-    trace_path_ = tp;
-    const char* at = std::getenv("HYKKT_SOLVE_TRACE_AT");
-    trace_at_ = at ? std::atol(at) : 20;
-    trace_items_.clear();
-    for (auto* v : {&fwd, &fwd_top, &bwd_top, &bwd}) trace_items_.insert(trace_items_.end(), v->begin(), v->end());
-    trace_.alloc(4 * static_cast<size_t>(sd_.n_fwd + sd_.n_fwd_top + sd_.n_bwd_top + sd_.n_bwd) + 4);
-  }
   dev::fill_i(flags_.p, 0, 1, st_);
 }
 
-void CholEngine::zero_L() { dev::fill(L_.p, 0.0, static_cast<int64_t>(L_.n), st_); }
-
-void CholEngine::assemble(const double* vals)
+void CholEngine::zero_L(const dev::Batch& b)
 {
-  zero_L();
-  dev::scatter(L_.p, a_dst_.p, vals, a_src_.p, n_assembled(), st_);
+  dev::fill(cd_.L, 0.0, pl_.l_off[pl_.ns], b, st_);
 }
 
-void CholEngine::shift_diag(double delta)
+void CholEngine::assemble(const double* vals, const dev::Batch& b)
 {
-  if (delta != 0.0) dev::add_at(L_.p, diag_pos_.p, pl_.n, delta, st_);
+  zero_L(b);
+  dev::scatter(cd_.L, a_dst_.p, vals, a_src_.p, n_assembled(), b, st_);
 }
 
-void CholEngine::factor(double pivot_tol)
+void CholEngine::shift_diag(double delta, const dev::Batch& b)
 {
+  if (delta != 0.0) dev::add_at(cd_.L, diag_pos_.p, pl_.n, delta, b, st_);
+}
+
+void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
+{
+  HYKKT_REQUIRE(cd_.L && bt.count <= cap_, HYKKT_E_STATE, "factor storage not bound");
+  const dev::Batch b{bt.count, vs_, nullptr};
   cd_.pivot_tol = pivot_tol;
-  dev::fill_i(pivot_.p, INT_MAX, 1, st_);
+  dev::fill_i(cd_.pivot, INT_MAX, b.count, st_);
   Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
-  // Bracket one launch (skipped when it has no tasks, like the launchers).
+  // Bracket one launch (skipped when it has no tasks, like the launchers);
+  // charged with the work of all the systems it covers.
   auto timed_on = [&](hipStream_t st, int kind, const Launch& la, auto&& launch) {
     if (la.cnt <= 0) return;
     if (pf) pf->begin(st);
     launch();
-    if (pf) pf->end(kind, st, la.flops, la.bytes);
+    if (pf) pf->end(kind, st, la.flops * b.count, la.bytes * b.count);
   };
   auto timed = [&](int kind, const Launch& la, auto&& launch) { timed_on(st_, kind, la, launch); };
   for (const Level& L : lv_) {
     timed(PK_EXTEND_ADD, L.ea_panel, [&] {
       dev::extend_add(cd_, L.ea_panel.cnt, t_ea_p_.p + L.ea_panel.off,
                       t_ea_col_.p + L.ea_panel.off, t_ea_cptr_.p + L.ea_panel.off, t_ea_cc_.p,
-                      t_ea_ccol_.p, st_);
+                      t_ea_ccol_.p, b, st_);
     });
     // Blocked steps with look-ahead.  Main stream: diagonal blocks, TRSM,
     // update of the next block column(s); second stream: the rest of the
@@ -495,12 +526,12 @@ void CholEngine::factor(double pivot_tol)
       const int umode = kblk == 1 ? 0 : kblk + 1, ustep = a;
       const int upd_hi = (last && G > 1) ? step + G : step + 1;
       timed(PK_POTRF, L.potrf[j], [&] {
-        dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb,
+        dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb, b,
                        st_);
       });
       timed(PK_DENSE_NT, L.trsm[j], [&] {
         dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
-                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, st_,
+                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, b, st_,
                       L.trsm[j].short_rows);
       });
       if (rest_pending && upd_hi >= rest_lo) {
@@ -511,7 +542,7 @@ void CholEngine::factor(double pivot_tol)
       timed(PK_DENSE_NT, L.upd_next[j], [&] {
         dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
                       t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, umode,
-                      ustep, st_, L.upd_next[j].short_rows);
+                      ustep, b, st_, L.upd_next[j].short_rows);
       });
       if (L.upd_rest[j].cnt > 0) {
         HIPCHK(hipEventRecord(ev_main_, st_));
@@ -519,7 +550,7 @@ void CholEngine::factor(double pivot_tol)
         timed_on(st2_, PK_DENSE_NT, L.upd_rest[j], [&] {
           dev::dense_nt(cd_, L.upd_rest[j].cnt, t_upd_sn_.p + L.upd_rest[j].off,
                         t_upd_tr_.p + L.upd_rest[j].off, t_upd_tc_.p + L.upd_rest[j].off, umode,
-                        ustep, st2_);
+                        ustep, b, st2_);
         });
         HIPCHK(hipEventRecord(ev_rest_, st2_));
         rest_pending = true;
@@ -529,21 +560,23 @@ void CholEngine::factor(double pivot_tol)
     if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
     timed(PK_DENSE_NT, L.syrk, [&] {
       dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
-                    t_syrk_tc_.p + L.syrk.off, 1, 0, st_);
+                    t_syrk_tc_.p + L.syrk.off, 1, 0, b, st_);
     });
     timed(PK_EXTEND_ADD, L.ea_u, [&] {
       dev::extend_add(cd_, L.ea_u.cnt, t_ea_p_.p + L.ea_u.off, t_ea_col_.p + L.ea_u.off,
-                      t_ea_cptr_.p + L.ea_u.off, t_ea_cc_.p, t_ea_ccol_.p, st_);
+                      t_ea_cptr_.p + L.ea_u.off, t_ea_cc_.p, t_ea_ccol_.p, b, st_);
     });
   }
 }
 
-int CholEngine::pivot()
+std::vector<int> CholEngine::pivots(int count)
 {
-  int h = INT_MAX;
-  HIPCHK(hipMemcpyAsync(&h, pivot_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+  std::vector<int> h(std::max(count, 1), INT_MAX);
+  HIPCHK(hipMemcpyAsync(h.data(), cd_.pivot, sizeof(int) * count, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  return h == INT_MAX ? -1 : h;
+  for (int& v : h) v = (v == INT_MAX) ? -1 : v;
+  h.resize(count);
+  return h;
 }
 
 int CholEngine::solve_error()
@@ -555,49 +588,34 @@ int CholEngine::solve_error()
   return h;
 }
 
-void CholEngine::solve(double* x)
+void CholEngine::solve(double* x, const dev::Batch& bt)
 {
+  const dev::Batch b{bt.count, vs_, bt.mask};
   Profiler* pf = prof_ && prof_->on() ? prof_ : nullptr;
-  const int64_t nu = static_cast<int64_t>(ubuf_.n);
+  const int64_t nu = n_ubuf_;
   // Algorithmic, per part of the tree: each sweep reads that part's factor
   // entries (lower trapezoids of its panels) once, 2 flops per entry, and
-  // reads + writes its vector entries.
+  // reads + writes its vector entries; charged for every system of the
+  // launch (those a mask skips included).
   auto sweep = [&](int kind, int part, bool fwd) {
     if ((part == 0 ? (fwd ? sd_.n_fwd : sd_.n_bwd) : (fwd ? sd_.n_fwd_top : sd_.n_bwd_top)) <= 0)
       return;
     if (pf) pf->begin(st_);
-    if (fwd) dev::solve_forward(cd_, sd_, x, ybuf_.p, part, st_);
-    else dev::solve_backward(cd_, sd_, ybuf_.p, x, part, st_);
+    if (fwd) dev::solve_forward(cd_, sd_, x, ybuf_, part, b, st_);
+    else dev::solve_backward(cd_, sd_, ybuf_, x, part, b, st_);
     if (pf)
-      pf->end(kind, st_, 2.0 * solve_lower_[part], 8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]));
+      pf->end(kind, st_, 2.0 * solve_lower_[part] * b.count,
+              8.0 * (solve_lower_[part] + 2.0 * solve_n_[part]) * b.count);
   };
   // forward: b in x -> y in ybuf; backward: y in ybuf -> x.  x holds b until
   // the forward sweep is done, so it is filled after it; the other buffers
   // (and the item counters) before it, in one launch.
-  dev::fill_sentinel3(ybuf_.p, pl_.n, ubuf_.p, nu, bpart_.p, n_bpart_, st_, sd_.ticket);
+  dev::fill_sentinel3(ybuf_, pl_.n, sd_.ubuf, nu, sd_.bpart, n_bpart_, b, st_, sd_.ticket);
   sweep(PK_SOLVE_FWD_FOREST, 0, true);
   sweep(PK_SOLVE_FWD_TOP, 1, true);
-  dev::fill_sentinel(x, pl_.n, st_);
+  dev::fill_sentinel(x, pl_.n, b, st_);
   sweep(PK_SOLVE_BWD_TOP, 1, false);
   sweep(PK_SOLVE_BWD_FOREST, 0, false);
-  if (trace_at_ >= 0) {
-    // the development timeline: stamps of the solve after the marked one
-    sd_.trace = (n_solve_calls_ == trace_at_) ? trace_.p : nullptr;
-    if (n_solve_calls_ == trace_at_ + 1) {
-      std::vector<long long> h(trace_.n);
-      HIPCHK(hipMemcpyAsync(h.data(), trace_.p, h.size() * sizeof(long long),
-                            hipMemcpyDeviceToHost, st_));
-      HIPCHK(hipStreamSynchronize(st_));
-      if (FILE* f = std::fopen(trace_path_.c_str(), "wb")) {
-        const int cnt[4] = {sd_.n_fwd, sd_.n_fwd_top, sd_.n_bwd_top, sd_.n_bwd};
-        std::fwrite(cnt, sizeof(int), 4, f);
-        std::fwrite(trace_items_.data(), sizeof(int), trace_items_.size(), f);
-        std::fwrite(h.data(), sizeof(long long), h.size() - 4, f);
-        std::fclose(f);
-      }
-    }
-    n_solve_calls_++;
-  }
 }
 
 }  // namespace hykkt

@@ -58,32 +58,57 @@ struct DevBuf {
 };
 
 // Numeric supernodal multifrontal Cholesky on the device for one sparsity
-// pattern (the analysis is done once; factor() and solve() run per solve).
+// pattern (the analysis is done once; factor() and solve() run per solve),
+// for up to `capacity` systems of that pattern at once.  The per-system
+// storage (factor panels, update matrices, inverted diagonal blocks, sweep
+// buffers) is either bound into a caller's arena (bind: system k's copy
+// k*vs doubles from system 0's) or owned (own).
 class CholEngine {
  public:
+  // Doubles of per-system storage, by kind.
+  struct Sizes {
+    int64_t L, U, dinv, y, ubuf, bpart;
+  };
+  // Per-system storage of system 0 (pivot: one int per system).
+  struct Storage {
+    double *L, *U, *dinv, *y, *ubuf, *bpart;
+    int* pivot;
+  };
   // `a`: full n x n CSR whose lower triangle is factored.
   CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st);
   ~CholEngine();
   CholEngine(const CholEngine&) = delete;
   CholEngine& operator=(const CholEngine&) = delete;
   const CholPlan& plan() const { return pl_; }
-  double* L() { return L_.p; }
+  Sizes sizes() const;
+  // Use storage for `cap` systems, vs doubles apart (the inverted diagonal
+  // blocks are zeroed here: only their lower parts are ever written).
+  void bind(const Storage& s, int64_t vs, int cap);
+  // Allocate and bind storage for `cap` systems.
+  void own(int cap);
+  int64_t stride() const { return vs_; }
+  double* L() { return cd_.L; }
   // Zero the factor storage (before assembly of the matrix entries).
-  void zero_L();
+  void zero_L(const dev::Batch& b);
   // Scatter values (in a's CSR order) into the factor storage (zeroes first).
-  void assemble(const double* vals_dev);
+  void assemble(const double* vals_dev, const dev::Batch& b);
   // Add `delta` to every diagonal entry of the assembled matrix.
-  void shift_diag(double delta);
-  // Factor the assembled matrix; pivot failures are recorded on the device.
-  void factor(double pivot_tol);
-  // First failing pivot (internal index) or -1; synchronizes the stream.
-  int pivot();
+  void shift_diag(double delta, const dev::Batch& b);
+  // Factor the assembled matrices of b.count systems; pivot failures are
+  // recorded on the device.
+  void factor(double pivot_tol, const dev::Batch& b);
+  // First failing pivot (internal index) or -1 of each of `count` systems;
+  // synchronizes the stream.
+  std::vector<int> pivots(int count);
+  int pivot() { return pivots(1)[0]; }
   // Nonzero if a triangular-solve sweep hit its wait bound since the last
   // call (synchronizes the stream, clears the error word; solves in between
   // leave it set).
   int solve_error();
-  // Solve L L^T x = b in place, x in internal (postordered) order.
-  void solve(double* x);
+  // Solve L L^T x = b in place for each system of b (its mask skips
+  // systems), x in internal (postordered) order, per system at x + k*b.vs.
+  void solve(double* x, const dev::Batch& b);
+  void solve(double* x) { solve(x, dev::Batch{1, vs_, nullptr}); }
   // Device arrays of the assembly map (a's lower-triangle entries).
   const int* a_src() const { return a_src_.p; }
   const int64_t* a_dst() const { return a_dst_.p; }
@@ -110,6 +135,10 @@ class CholEngine {
 
   CholPlan pl_;
   hipStream_t st_;
+  int64_t vs_ = 0;   // per-system stride of the bound storage
+  int cap_ = 0;
+  DevBuf<double> own_;   // owned storage (own)
+  DevBuf<int> own_piv_;
   hipStream_t st2_ = nullptr;   // look-ahead: the rest of each trailing update
   hipEvent_t ev_main_ = nullptr, ev_rest_ = nullptr;
   Profiler* prof_ = nullptr;
@@ -118,8 +147,7 @@ class CholEngine {
   DevBuf<int> sn_first_, rows_ptr_, rows_, rel_, child_ptr_, child_list_, post_, sn_parent_;
   DevBuf<int64_t> l_off_, u_off_, a_dst_, diag_pos_, uvec_off_, dinv_off_, front_off_;
   DevBuf<int> a_src_;
-  DevBuf<double> L_, U_, ubuf_, dinv_;
-  DevBuf<int> pivot_;
+  int64_t n_ubuf_ = 0, n_dinv_ = 0;
   // task lists
   std::vector<Level> lv_;
   DevBuf<int> t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
@@ -129,15 +157,8 @@ class CholEngine {
   // solve items
   DevBuf<int> fwd_items_, bwd_items_, pull_ptr_, pull_src_, flags_;
   DevBuf<int64_t> bpart_off_;
-  DevBuf<double> bpart_;
   DevBuf<int> fwd_top_items_, bwd_top_items_;
-  DevBuf<double> ybuf_;                   // forward-sweep output
-  // development timeline of one solve (HYKKT_SOLVE_TRACE=file, taken at the
-  // HYKKT_SOLVE_TRACE_AT-th solve, default 20)
-  DevBuf<long long> trace_;
-  std::vector<int> trace_items_;
-  std::string trace_path_;
-  long trace_at_ = -1, n_solve_calls_ = 0;
+  double* ybuf_ = nullptr;   // forward-sweep output (bound storage)
   int64_t n_bpart_ = 0;
   double solve_lower_[2] = {0.0, 0.0};   // stored lower-trapezoid entries: forest, top
   int64_t solve_n_[2] = {0, 0};          // columns: forest, top
@@ -145,25 +166,42 @@ class CholEngine {
 
 // The Schur-complement operator S = Jh A^{-1} Jh^T with A factored by `chol`
 // (Jh: m x n, columns in the factor's internal order) and the CG workspace.
+// Per-system vectors are at their system-0 address + k * vs (the Batch);
+// the flag arrays hold one int per system.
 struct SchurOps {
   int m = 0, n = 0;
-  const int *jh_rp = nullptr, *jh_ci = nullptr;
+  const int *jh_rp = nullptr, *jh_ci = nullptr, *jh_blk = nullptr;
+  int jh_nblk = 0;
   const double* jh_v = nullptr;
-  const int *jth_rp = nullptr, *jth_ci = nullptr;
+  const int *jth_rp = nullptr, *jth_ci = nullptr, *jth_blk = nullptr;
+  int jth_nblk = 0;
   const double* jth_v = nullptr;
   CholEngine* chol = nullptr;
   hipStream_t st = nullptr;
   double *t1 = nullptr;                                            // n
   double *x = nullptr, *r = nullptr, *w = nullptr, *p = nullptr, *s = nullptr;   // m
-  double *sc = nullptr, *part = nullptr;
+  double *sc = nullptr, *part = nullptr;   // 8 scalars; partials (max of both reductions)
+  int *live = nullptr, *iters = nullptr, *status = nullptr;   // per system
+  double* gath = nullptr;   // 2 doubles per system: scalars gathered for the readback
+};
+// Partials a SchurOps needs per system.
+int schur_partials(int m, int jh_nblk);
+
+struct CgResult {
+  int iters = 0, status = 0;
+  double final_norm = 0.0;
 };
 
-// SchurComplementConjugateGradient::solve (…ConjugateGradient.cpp:89-194) on
-// the device: ops.x holds x0 on entry and the solution on exit.  The host
-// reads ||r||^2 back every `check_every` iterations for the stopping test.
-// Returns 0 if converged (sqrt(r.r) < tol), 1 after itmax iterations.
-// x0_zero: ops.x is known to be all zeros (r = b without a solve).
-int schur_cg_run(SchurOps& ops, const double* b, double tol, int itmax, int check_every,
-                 int* iters, double* final_norm, bool x0_zero = false);
+// SchurComplementConjugateGradient::solve (…ConjugateGradient.cpp:89-194)
+// on the device for the systems of `b` (its mask: the systems taking part):
+// ops.x holds x0 on entry and the solution on exit.  Every system's stopping
+// test (sqrt(r.r) < tol, or itmax iterations) is taken on the device, which
+// then leaves that system alone; the host reads the systems' live flags
+// only after `chunk` iterations and then after every `chunk_next`, so the
+// iterations in between run without a host round trip.  x0_zero: every x is
+// zero (r = b without a solve).  Returns per system (count entries).
+std::vector<CgResult> schur_cg_run(SchurOps& ops, const double* b, double tol, int itmax,
+                                   const dev::Batch& bt, bool x0_zero, int chunk,
+                                   int chunk_next = 1);
 
 }  // namespace hykkt

@@ -9,6 +9,7 @@
 
 #include <climits>
 
+#include "batch_dev.hpp"
 #include "kernels.hpp"
 
 namespace hykkt {
@@ -21,43 +22,47 @@ struct SnInfo {
   double* Ls;
 };
 
-__device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
+__device__ __forceinline__ SnInfo sninfo(const CholDev& c, double* L, int s)
 {
   SnInfo i;
   i.a = c.sn_first[s];
   i.nc = c.sn_first[s + 1] - i.a;
   i.nr = c.rows_ptr[s + 1] - c.rows_ptr[s];
   i.f = i.nc + i.nr;
-  i.Ls = c.L + c.l_off[s];
+  i.Ls = L + c.l_off[s];
   return i;
 }
 
 // ----------------------------------------------------------------- extend-add
-// One workgroup per parent front column.  Children are added one after the
-// other (fixed order => deterministic sums); within one child the relative
-// index map is injective.
+// One workgroup per parent front column and system.  Children are added one
+// after the other (fixed order => deterministic sums); within one child the
+// relative index map is injective.
 __global__ void __launch_bounds__(256) k_extend_add(CholDev c, const int* __restrict__ task_p,
                                                     const int* __restrict__ task_col,
                                                     const int* __restrict__ cptr,
                                                     const int* __restrict__ cc,
-                                                    const int* __restrict__ ccol)
+                                                    const int* __restrict__ ccol, Batch b)
 {
+  const int sy = batch_sys(b);
+  if (sy < 0) return;
+  double* const L = batch_at(c.L, sy, b);
+  double* const U = batch_at(c.U, sy, b);
   const int t = blockIdx.x;
   const int p = task_p[t], col = task_col[t];
-  const SnInfo ip = sninfo(c, p);
+  const SnInfo ip = sninfo(c, L, p);
   double* base;
   int shift;
   if (col < ip.nc) {
     base = ip.Ls + static_cast<int64_t>(col) * ip.f;
     shift = 0;
   } else {
-    base = c.U + c.u_off[p] + packed_lower(0, col - ip.nc, ip.nr);   // rows >= col only
+    base = U + c.u_off[p] + packed_lower(0, col - ip.nc, ip.nr);   // rows >= col only
     shift = -ip.nc;
   }
   for (int k = cptr[t]; k < cptr[t + 1]; k++) {
     const int ch = cc[k], r2 = ccol[k];
     const int nrc = c.rows_ptr[ch + 1] - c.rows_ptr[ch];
-    const double* src = c.U + c.u_off[ch] + packed_lower(0, r2, nrc);   // rows r1 >= r2
+    const double* src = U + c.u_off[ch] + packed_lower(0, r2, nrc);   // rows r1 >= r2
     const int* relc = c.rel + c.rows_ptr[ch];
     for (int r1 = r2 + threadIdx.x; r1 < nrc; r1 += blockDim.x) base[relc[r1] + shift] += src[r1];
     __syncthreads();
@@ -68,10 +73,10 @@ __global__ void __launch_bounds__(256) k_extend_add(CholDev c, const int* __rest
 
 // =================================================================== launchers
 void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task_col,
-                const int* cptr, const int* cc, const int* ccol, hipStream_t st)
+                const int* cptr, const int* cc, const int* ccol, const Batch& b, hipStream_t st)
 {
-  if (ntasks <= 0) return;
-  k_extend_add<<<ntasks, 256, 0, st>>>(c, task_p, task_col, cptr, cc, ccol);
+  if (ntasks <= 0 || b.count <= 0) return;
+  k_extend_add<<<dim3(ntasks, b.count), 256, 0, st>>>(c, task_p, task_col, cptr, cc, ccol, b);
   HIPCHK(hipGetLastError());
 }
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "batch_dev.hpp"
 #include "kernels.hpp"
 
 namespace hykkt {
@@ -171,14 +172,14 @@ struct SnInfo {
   double* Ls;
 };
 
-__device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
+__device__ __forceinline__ SnInfo sninfo(const CholDev& c, double* L, int s)
 {
   SnInfo i;
   i.a = c.sn_first[s];
   i.nc = c.sn_first[s + 1] - i.a;
   i.nr = c.rows_ptr[s + 1] - c.rows_ptr[s];
   i.f = i.nc + i.nr;
-  i.Ls = c.L + c.l_off[s];
+  i.Ls = L + c.l_off[s];
   return i;
 }
 
@@ -189,14 +190,17 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
 // TM = 128: 128x128 tiles (all modes).  TM = 32: 32-row x 128-column tiles
 // (modes 0 and 2), four times as many workgroups for the short launches on
 // the critical path (the TRSM and the next-column update of a step with few
-// tiles).
+// tiles).  System blockIdx.y.
 template <int TM>
 __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __restrict__ tsn,
                                                   const int* __restrict__ ttr,
-                                                  const int* __restrict__ ttc, int mode, int step)
+                                                  const int* __restrict__ ttc, int mode, int step,
+                                                  Batch bt)
 {
   extern __shared__ double smem[];
-  const SnInfo si = sninfo(c, tsn[blockIdx.x]);
+  const int sy = batch_sys(bt);
+  if (sy < 0) return;
+  const SnInfo si = sninfo(c, batch_at(c.L, sy, bt), tsn[blockIdx.x]);
   const int tr = ttr[blockIdx.x], tc = ttc[blockIdx.x];
   GemmArgs g;
   if (mode == 1) {
@@ -205,7 +209,7 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
     g.A = L21 + R0;
     g.B = L21 + C0;
     g.lda = g.ldb = si.f;
-    g.C = c.U + c.u_off[tsn[blockIdx.x]];   // packed lower triangle of U
+    g.C = batch_at(c.U, sy, bt) + c.u_off[tsn[blockIdx.x]];   // packed lower triangle of U
     g.ldc = si.nr;
     g.pk = si.nr;
     g.pr0 = R0;
@@ -237,7 +241,8 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
       const int R0 = cs + tr * TM;
       g.A = panel + R0;
       g.lda = si.f;
-      g.B = c.dinv + c.dinv_off[tsn[blockIdx.x]] + static_cast<int64_t>(step) * BT * BT;
+      g.B = batch_at(c.dinv, sy, bt) + c.dinv_off[tsn[blockIdx.x]] +
+            static_cast<int64_t>(step) * BT * BT;
       g.ldb = BT;
       g.C = panel + R0;
       g.ldc = si.f;
@@ -403,10 +408,11 @@ __device__ __forceinline__ void lower_entry(int e, int& r, int& col)
 // Diagonal block `step` of supernode s, by one workgroup of 256 threads with
 // smem >= potrf_smem<BK>().  BK: padded block edge (32, 64 or 128) >= b.
 template <int BK, bool COH>
-__device__ __forceinline__ void potrf_block(const CholDev& c, int s, int step, double* smem)
+__device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double* dinv, int* pivot,
+                                            int s, int step, double* smem)
 {
   constexpr int NPN = BK / PW;   // panels
-  const SnInfo si = sninfo(c, s);
+  const SnInfo si = sninfo(c, L, s);
   const int k0 = step * BT;
   if (k0 >= si.nc) return;
   const int b = min(BK, si.nc - k0);
@@ -583,7 +589,7 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, int s, int step, d
   // the inverse into the 128 x 128 dinv slot (column-major; entries outside
   // the b x b lower triangle stay at the zeros written when the slot was
   // created), over the folded lower triangle, LDS reads first so they overlap
-  double* W = c.dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
+  double* W = dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
   constexpr int SB2 = (PER + 1) / 2;   // entries per batch
 #pragma unroll
   for (int u0 = 0; u0 < PER; u0 += SB2) {
@@ -602,30 +608,34 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, int s, int step, d
       if (u0 + u < PER && e < NL && r < b) stg<COH>(W + static_cast<int64_t>(col) * BT + r, xv[u]);
     }
   }
-  if (tid == 0 && bad != INT_MAX) atomicMin(c.pivot, bad);
+  if (tid == 0 && bad != INT_MAX) atomicMin(pivot, bad);
 }
 
 template <int BK>
 __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
-                                                   int step)
+                                                   int step, Batch bt)
 {
   extern __shared__ double smem[];
+  const int sy = batch_sys(bt);
+  if (sy < 0) return;
   const int s = sns[blockIdx.x];
   if (step * BT >= c.sn_first[s + 1] - c.sn_first[s]) return;
-  potrf_block<BK, false>(c, s, step, smem);
+  potrf_block<BK, false>(c, batch_at(c.L, sy, bt), batch_at(c.dinv, sy, bt), c.pivot + sy, s,
+                         step, smem);
 }
 
 }  // namespace
 
 size_t dense_nt_smem() { return 2 * KC * LDA_S * sizeof(double); }
 void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, const int* ttc,
-              int mode, int step, hipStream_t st, bool short_rows)
+              int mode, int step, const Batch& b, hipStream_t st, bool short_rows)
 {
-  if (ntasks <= 0) return;
+  if (ntasks <= 0 || b.count <= 0) return;
+  const dim3 grid(ntasks, b.count);
   if (short_rows)
-    k_dense_nt<32><<<ntasks, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step);
+    k_dense_nt<32><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
   else
-    k_dense_nt<128><<<ntasks, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step);
+    k_dense_nt<128><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
   HIPCHK(hipGetLastError());
 }
 
@@ -637,7 +647,8 @@ static size_t potrf_smem()
 }
 
 template <int BK>
-static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hipStream_t st)
+static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, const Batch& b,
+                         hipStream_t st)
 {
   // set once per process; a function-local static initialises thread-safely
   static const bool attr = [] {
@@ -647,16 +658,17 @@ static void launch_potrf(const CholDev& c, int nsn, const int* sns, int step, hi
     return true;
   }();
   (void)attr;
-  k_potrf_inv<BK><<<nsn, 256, potrf_smem<BK>(), st>>>(c, sns, step);
+  k_potrf_inv<BK><<<dim3(nsn, b.count), 256, potrf_smem<BK>(), st>>>(c, sns, step, b);
   HIPCHK(hipGetLastError());
 }
 
-void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, hipStream_t st)
+void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, const Batch& b,
+               hipStream_t st)
 {
-  if (nsn <= 0) return;
-  if (max_b <= 32) launch_potrf<32>(c, nsn, sns, step, st);
-  else if (max_b <= 64) launch_potrf<64>(c, nsn, sns, step, st);
-  else launch_potrf<128>(c, nsn, sns, step, st);
+  if (nsn <= 0 || b.count <= 0) return;
+  if (max_b <= 32) launch_potrf<32>(c, nsn, sns, step, b, st);
+  else if (max_b <= 64) launch_potrf<64>(c, nsn, sns, step, b, st);
+  else launch_potrf<128>(c, nsn, sns, step, b, st);
 }
 
 }  // namespace dev

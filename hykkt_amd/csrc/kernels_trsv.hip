@@ -33,6 +33,11 @@
 // (SPIN_LIMIT); on expiry the item sets an error word the host reports, and
 // carries on, so the kernel always terminates.
 //
+// Several systems of one pattern share a launch: launch item i*count + k is
+// item i of system k, whose buffers (factor, right-hand side, exchanged
+// values) sit k*vs doubles from system 0's.  The order property above holds
+// for the combined numbering, and a system the mask skips returns at once.
+//
 // Synchronisation is by the data itself.  Every value passed between items
 // (the forward y blocks, update vectors, backward chunk partials, backward x
 // blocks) lives in a buffer that is filled with a sentinel NaN bit pattern
@@ -47,6 +52,7 @@
 
 #include <algorithm>
 
+#include "batch_dev.hpp"
 #include "kernels.hpp"
 
 namespace hykkt {
@@ -79,8 +85,8 @@ __device__ __forceinline__ int64_t uni64(int64_t v)
   return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// Supernode s (wave-uniform) of the factor.
-__device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
+// Supernode s (wave-uniform) of the factor L.
+__device__ __forceinline__ SnInfo sninfo(const CholDev& c, const double* L, int s)
 {
   SnInfo i;
   i.a = uni(c.sn_first[s]);
@@ -89,8 +95,25 @@ __device__ __forceinline__ SnInfo sninfo(const CholDev& c, int s)
   i.f = i.nc + i.nr;
   i.np = (i.nc + SB - 1) / SB;
   i.nu = (i.nr + SB - 1) / SB;
-  i.Ls = c.L + uni64(c.l_off[s]);
+  i.Ls = L + uni64(c.l_off[s]);
   return i;
+}
+
+// One system's buffers in a sweep (wave-uniform pointers).
+struct SysBufs {
+  const double* L;
+  const double* dinv;
+  double* ubuf;
+  double* bpart;
+  const double* in;    // right-hand side (forward) or y (backward)
+  double* out;         // y (forward) or x (backward)
+};
+
+__device__ __forceinline__ SysBufs sys_bufs(const CholDev& c, const SolveDev& sd, int sy,
+                                            const Batch& bt, const double* in, double* out)
+{
+  const int64_t o = static_cast<int64_t>(sy) * bt.vs;
+  return SysBufs{c.L + o, c.dinv + o, sd.ubuf + o, sd.bpart + o, in + o, out + o};
 }
 
 // Sentinel: a signalling-NaN payload no arithmetic result carries.
@@ -145,12 +168,6 @@ __device__ __forceinline__ int take_item(int* ticket)
   return uni(item);
 }
 
-// Stamp k of item `item` in the development timeline (when one is bound).
-__device__ __forceinline__ void stamp(const SolveDev& sd, int item, int k)
-{
-  if (sd.trace && threadIdx.x == 0) sd.trace[4 * static_cast<int64_t>(item) + k] = wall_clock64();
-}
-
 // Fixed-order sum of the NQ slice partials of one row (column).
 template <int NQ>
 __device__ __forceinline__ double sum_slices(const double (&part)[NQ][SB], int i)
@@ -185,10 +202,15 @@ __device__ __forceinline__ double ld(rsrc_t r, uint32_t voff, uint32_t soff)
 // p0[0..n0) p1[0..n1) p2[0..n2) takes the sentinel.
 __global__ void k_fill_sentinel(double* __restrict__ p0, int64_t n0, double* __restrict__ p1,
                                 int64_t n1, double* __restrict__ p2, int64_t n2,
-                                int* __restrict__ zero4)
+                                int* __restrict__ zero4, Batch bt)
 {
-  if (zero4 && blockIdx.x == 0 && threadIdx.x < 4)
+  if (zero4 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4)
     __hip_atomic_store(zero4 + threadIdx.x, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int sy = batch_sys(bt);
+  if (sy < 0) return;
+  p0 = batch_at(p0, sy, bt);
+  p1 = batch_at(p1, sy, bt);
+  p2 = batch_at(p2, sy, bt);
   const int64_t n = n0 + n1 + n2;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -249,19 +271,22 @@ __device__ __forceinline__ double slice_dot_stream(rsrc_t rs, uint32_t vo, uint3
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
-                                                  const double* __restrict__ b,
-                                                  double* __restrict__ y)
+                                                  const double* __restrict__ rhs,
+                                                  double* __restrict__ yout, Batch bt)
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double yv[SB];
   __shared__ double part[NQ][SB];
-  const int it = take_item(sd.ticket + (PF ? 1 : 0));
+  const int itc = take_item(sd.ticket + (PF ? 1 : 0));
+  const int sy = itc % bt.count, it = itc / bt.count;
+  if (bt.mask && !bt.mask[sy]) return;
+  const SysBufs B = sys_bufs(c, sd, sy, bt, rhs, yout);
+  const double* b = B.in;
+  double* y = B.out;
   const int s = uni(items[3 * it]);
   const int kind = uni(items[3 * it + 1]);
   const int blk = uni(items[3 * it + 2]);
-  const int tid = (PF ? sd.n_fwd : 0) + it;   // timeline slot
-  stamp(sd, tid, 0);
-  const SnInfo si = sninfo(c, s);
+  const SnInfo si = sninfo(c, B.L, s);
   const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
   const int rend = (kind == 0) ? min(si.nc, r0 + SB) : min(si.f, r0 + SB);
   const int nrows = rend - r0;
@@ -281,7 +306,7 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   const uint32_t vo = static_cast<uint32_t>(r) * 8u;
   const uint32_t so = static_cast<uint32_t>(q * QW) * f8;
   // row `row` of the inverted diagonal block (upper part stored as zeros)
-  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+  const rsrc_t wv = view(B.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(row) * 8u, wso = static_cast<uint32_t>(q * QW) * SB * 8u;
 
@@ -300,9 +325,8 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     v = (kind == 0) ? b[si.a + r] : 0.0;
     const int64_t fr = sd.front_off[s] + r;
     for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
-      v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
+      v += spin_load(B.ubuf + sd.pull_src[p], sd.err);
   }
-  stamp(sd, tid, 1);
   // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready
   for (int j = 0; j < jmax; j++) {
     const int bj = min(SB, si.nc - j * SB);
@@ -321,7 +345,6 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     if (q == 0) v -= sum_slices<NQ>(part, row);
     __syncthreads();
   }
-  stamp(sd, tid, 2);
   if (kind == 0) {
     // y = Dinv_blk v
     if (q == 0) yv[row] = live ? v : 0.0;
@@ -333,15 +356,14 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     __syncthreads();
     if (q == 0 && live) xstore(y + si.a + r, sum_slices<NQ>(part, row));
   } else {
-    if (q == 0 && live) xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v);
+    if (q == 0 && live) xstore(B.ubuf + sd.uvec_off[s] + (r - si.nc), v);
   }
-  stamp(sd, tid, 3);
 }
 
 // ------------------------------------------------------- forest, persistent
 // The forward sweep of the small-supernode forest runs on resident
-// workgroups that loop over items.  A workgroup takes its next item as soon as the current one has all
-// its inputs (past its waits): it never holds an item it cannot start, so
+// workgroups that loop over items.  A workgroup takes its next item as soon as the current one has
+// all its inputs (past its waits): it never holds an item it cannot start, so
 // the smallest unfinished item is still always being worked on, and the
 // counter's round trip overlaps the item's arithmetic.
 __device__ __forceinline__ int ticket_add(int* ticket)
@@ -354,24 +376,35 @@ __device__ __forceinline__ int ticket_add(int* ticket)
 // is loaded before its waits.
 __global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd,
                                                           const int* __restrict__ items,
-                                                          const double* __restrict__ b,
-                                                          double* __restrict__ y)
+                                                          const double* __restrict__ rhs,
+                                                          double* __restrict__ yout, Batch bt)
 {
   constexpr int NQ = NT_FOREST / SB, QW = SB / NQ;
   __shared__ double yv[SB];
   __shared__ double part[NQ][SB];
   __shared__ int nxt;
   int* const ticket = sd.ticket + 0;
+  const int total = sd.n_fwd * bt.count;
   if (threadIdx.x == 0) nxt = ticket_add(ticket);
   __syncthreads();
-  int it = uni(nxt);
-  while (it < sd.n_fwd) {
+  int itc = uni(nxt);
+  while (itc < total) {
+    const int sy = itc % bt.count, it = itc / bt.count;
+    if (bt.mask && !bt.mask[sy]) {
+      // a skipped system: take the next item
+      __syncthreads();   // every thread has read nxt
+      if (threadIdx.x == 0) nxt = ticket_add(ticket);
+      __syncthreads();
+      itc = uni(nxt);
+      continue;
+    }
+    const SysBufs B = sys_bufs(c, sd, sy, bt, rhs, yout);
+    const double* b = B.in;
+    double* y = B.out;
     const int s = uni(items[3 * it]);
     const int kind = uni(items[3 * it + 1]);
     const int blk = uni(items[3 * it + 2]);
-    const int tid = it;   // timeline slot
-    stamp(sd, tid, 0);
-    const SnInfo si = sninfo(c, s);
+    const SnInfo si = sninfo(c, B.L, s);
     const int r0 = (kind == 0) ? blk * SB : si.nc + blk * SB;
     const int rend = (kind == 0) ? min(si.nc, r0 + SB) : min(si.f, r0 + SB);
     const int nrows = rend - r0;
@@ -383,7 +416,7 @@ __global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd
     double buf[QW];
     if (kind == 0) {
       // row `row` of the inverted diagonal block (upper part stored as zeros)
-      const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+      const rsrc_t wv = view(B.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                              SB * SB * 8u);
       slice_load<QW>(buf, wv, static_cast<uint32_t>(row) * 8u,
                      static_cast<uint32_t>(q * QW) * SB * 8u, SB * 8u, nrows - q * QW, live);
@@ -398,9 +431,8 @@ __global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd
       v = (kind == 0) ? b[si.a + r] : 0.0;
       const int64_t fr = sd.front_off[s] + r;
       for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
-        v += spin_load(sd.ubuf + sd.pull_src[p], sd.err);
+        v += spin_load(B.ubuf + sd.pull_src[p], sd.err);
     }
-    stamp(sd, tid, 1);
     if (kind == 1) {
       if (threadIdx.x < SB)
         yv[threadIdx.x] = (threadIdx.x < si.nc) ? spin_load(y + si.a + threadIdx.x, sd.err) : 0.0;
@@ -416,12 +448,11 @@ __global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd
     if (q == 0 && live) {
       const double t = sum_slices<NQ>(part, row);
       if (kind == 0) xstore(y + si.a + r, t);
-      else xstore(sd.ubuf + sd.uvec_off[s] + (r - si.nc), v - t);
+      else xstore(B.ubuf + sd.uvec_off[s] + (r - si.nc), v - t);
     }
-    stamp(sd, tid, 3);
     if (threadIdx.x == 0) nxt = nx;
     __syncthreads();
-    it = uni(nxt);
+    itc = uni(nxt);
   }
 }
 
@@ -473,16 +504,18 @@ __device__ __forceinline__ void column_sums(double (&acc)[16], double (*half)[SB
 //     bpart = L(rows, b-block)^T x(rows)
 //   pivot item: columns [128 blk, ...): x = Dinv^T (y - sum of its chunks'
 //     partials - L(later blocks, blk-block)^T x(later blocks))
-__device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& sd, int tslot,
-                                              int s, int kind, int idx, const double* __restrict__ y,
-                                              double* __restrict__ x, double* wl)
+__device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& sd,
+                                              const SysBufs& B, int s, int kind, int idx,
+                                              double* wl)
 {
   constexpr int QW = 16;
   __shared__ double xs[2][SB];
   __shared__ double half[2][SB];
   __shared__ double cs[SB];
   __shared__ double part[8][SB];
-  const SnInfo si = sninfo(c, s);
+  const double* __restrict__ y = B.in;
+  double* __restrict__ x = B.out;
+  const SnInfo si = sninfo(c, B.L, s);
   const int r = threadIdx.x & (SB - 1);
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
   const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
@@ -511,14 +544,12 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
       xs[0][threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
     }
     __syncthreads();
-    stamp(sd, tslot, 1);
     const double xr = xs[0][r];
 #pragma unroll
     for (int k = 0; k < QW; k++) acc[k] = l[k] * xr;
     column_sums(acc, half, cs);
-    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
+    double* out = B.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
     if (threadIdx.x < bb) xstore(out + threadIdx.x, cs[threadIdx.x]);
-    stamp(sd, tslot, 3);
     return;
   }
 
@@ -527,7 +558,7 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
   const int nb = min(SB, si.nc - c0);
   const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
   // x = Dinv^T t: x(i) = sum_j Dinv(j, i) t(j), thread (i = r, slice q of j)
-  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+  const rsrc_t wv = view(B.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   // The inverse goes to LDS up front (wl[j * 128 + i] = Dinv(j, i), read
   // along Dinv's column i), so the last link reads no global memory before
@@ -545,10 +576,9 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
   double t = 0.0;
   if (threadIdx.x < SB && threadIdx.x < nb) {
     t = y[si.a + c0 + threadIdx.x];
-    const double* bp = sd.bpart + sd.bpart_off[s] + c0 + threadIdx.x;
+    const double* bp = B.bpart + sd.bpart_off[s] + c0 + threadIdx.x;
     for (int uu = 0; uu < si.nu; uu++) t -= spin_load(bp + static_cast<int64_t>(uu) * si.nc, sd.err);
   }
-  stamp(sd, tslot, 1);
   for (int j = jtop; j > blk; j--) {
     const int bj = min(SB, si.nc - j * SB);
     double* xb = xs[j & 1];
@@ -560,7 +590,6 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
     for (int k = 0; k < QW; k++) acc[k] += l[k] * xr;
     if (j - 1 > blk) row_slice(l, (j - 1) * SB, min(SB, si.nc - (j - 1) * SB), c0, nb);
   }
-  stamp(sd, tslot, 2);
   column_sums(acc, half, cs);
   if (threadIdx.x < SB) xs[0][threadIdx.x] = (threadIdx.x < nb) ? t - cs[threadIdx.x] : 0.0;
   __syncthreads();
@@ -572,30 +601,32 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
   }
   __syncthreads();
   if (threadIdx.x < nb) xstore(x + si.a + c0 + threadIdx.x, sum_slices<8>(part, threadIdx.x));
-  stamp(sd, tslot, 3);
 }
 
 template <int NT, bool PF>
 __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
-                                                  const double* __restrict__ y,
-                                                  double* __restrict__ x)
+                                                  const double* __restrict__ yin,
+                                                  double* __restrict__ xout, Batch bt)
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
   __shared__ double xs[SB];
   __shared__ double part[NQ][SB];
-  const int it = take_item(sd.ticket + (PF ? 2 : 3));
+  const int itc = take_item(sd.ticket + (PF ? 2 : 3));
+  const int sy = itc % bt.count, it = itc / bt.count;
+  if (bt.mask && !bt.mask[sy]) return;
+  const SysBufs B = sys_bufs(c, sd, sy, bt, yin, xout);
   const int s = uni(items[3 * it]);
   const int kind = uni(items[3 * it + 1]);
   const int idx = uni(items[3 * it + 2]);
-  const int tid = sd.n_fwd + sd.n_fwd_top + (PF ? 0 : sd.n_bwd_top) + it;   // timeline slot
-  stamp(sd, tid, 0);
   if constexpr (PF) {
     extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS)
-    solve_bwd_top(c, sd, tid, s, kind, idx, y, x, wl);
+    solve_bwd_top(c, sd, B, s, kind, idx, wl);
     return;
   }
-  const SnInfo si = sninfo(c, s);
+  const double* __restrict__ y = B.in;
+  double* __restrict__ x = B.out;
+  const SnInfo si = sninfo(c, B.L, s);
   const int col = threadIdx.x & (SB - 1);
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // whole waves per slice
   const uint32_t f8 = static_cast<uint32_t>(si.f) * 8u;
@@ -621,15 +652,13 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     if (threadIdx.x < SB)
       xs[threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
     __syncthreads();
-    stamp(sd, tid, 1);
     double tp;
     if constexpr (PF) tp = slice_dot<QW>(l, xs + q * QW);
     else tp = slice_dot_stream<QW>(lv, cvo, so, 8u, nrows - q * QW, col < bb, xs + q * QW);
     part[q][col] = tp;
     __syncthreads();
-    double* out = sd.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
+    double* out = B.bpart + sd.bpart_off[s] + static_cast<int64_t>(u) * si.nc + b * SB;
     if (q == 0 && col < bb) xstore(out + col, sum_slices<NQ>(part, col));
-    stamp(sd, tid, 3);
     return;
   }
 
@@ -643,7 +672,7 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   auto lvalid = [&](int j) { return min(SB, si.nc - j * SB) - q * QW; };
   // Dinv_blk^T: x(col) = sum_{row >= col} W(row, col) t(row): column `col` of
   // W (its upper part is stored as zeros)
-  const rsrc_t wv = view(c.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
+  const rsrc_t wv = view(B.dinv + uni64(c.dinv_off[s]) + static_cast<int64_t>(blk) * SB * SB,
                          SB * SB * 8u);
   const uint32_t wvo = static_cast<uint32_t>(col) * SB * 8u, wso = static_cast<uint32_t>(q * QW) * 8u;
   const int jtop = si.np - 1;   // later pivot blocks j = jtop .. blk+1
@@ -657,10 +686,9 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   double t = 0.0;
   if (q == 0 && live) {
     t = y[si.a + c0 + col];
-    const double* bp = sd.bpart + sd.bpart_off[s] + c0 + col;
+    const double* bp = B.bpart + sd.bpart_off[s] + c0 + col;
     for (int u = 0; u < si.nu; u++) t -= spin_load(bp + static_cast<int64_t>(u) * si.nc, sd.err);
   }
-  stamp(sd, tid, 1);
   for (int j = jtop; j > blk; j--) {
     const int bj = min(SB, si.nc - j * SB);
     if (threadIdx.x < SB)
@@ -677,7 +705,6 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
     if (q == 0) t -= sum_slices<NQ>(part, col);
     __syncthreads();
   }
-  stamp(sd, tid, 2);
   if (q == 0) xs[col] = live ? t : 0.0;
   __syncthreads();
   double res;
@@ -686,28 +713,28 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   part[q][col] = res;
   __syncthreads();
   if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
-  stamp(sd, tid, 3);
 }
 
 }  // namespace
 
-void fill_sentinel(double* p, int64_t n, hipStream_t st, int* zero4)
+void fill_sentinel(double* p, int64_t n, const Batch& b, hipStream_t st, int* zero4)
 {
-  fill_sentinel3(p, n, nullptr, 0, nullptr, 0, st, zero4);
+  fill_sentinel3(p, n, nullptr, 0, nullptr, 0, b, st, zero4);
 }
 
 void fill_sentinel3(double* p0, int64_t n0, double* p1, int64_t n1, double* p2, int64_t n2,
-                    hipStream_t st, int* zero4)
+                    const Batch& b, hipStream_t st, int* zero4)
 {
   n0 = std::max<int64_t>(n0, 0);
   n1 = std::max<int64_t>(n1, 0);
   n2 = std::max<int64_t>(n2, 0);
   const int64_t n = n0 + n1 + n2;
-  if (n <= 0 && !zero4) return;
+  if ((n <= 0 && !zero4) || b.count <= 0) return;
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
   if (g < 1) g = 1;
-  k_fill_sentinel<<<static_cast<unsigned>(g), 256, 0, st>>>(p0, n0, p1, n1, p2, n2, zero4);
+  k_fill_sentinel<<<dim3(static_cast<unsigned>(g), b.count), 256, 0, st>>>(p0, n0, p1, n1, p2,
+                                                                             n2, zero4, b);
   HIPCHK(hipGetLastError());
 }
 
@@ -719,18 +746,22 @@ int forest_blocks_per_cu()
 }
 
 void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double* y, int part,
-                   hipStream_t st)
+                   const Batch& bt, hipStream_t st)
 {
+  if (bt.count <= 0) return;
   if (part == 0 && sd.n_fwd > 0)
-    k_fwd_forest<<<std::min(sd.n_fwd, sd.grid_fwd), NT_FOREST, 0, st>>>(c, sd, sd.fwd_items, b, y);
+    k_fwd_forest<<<std::min(sd.n_fwd * bt.count, sd.grid_fwd), NT_FOREST, 0, st>>>(
+        c, sd, sd.fwd_items, b, y, bt);
   if (part == 1 && sd.n_fwd_top > 0)
-    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, b, y);
+    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top * bt.count, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top,
+                                                                          b, y, bt);
   HIPCHK(hipGetLastError());
 }
 
 void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
-                    hipStream_t st)
+                    const Batch& bt, hipStream_t st)
 {
+  if (bt.count <= 0) return;
   if (part == 1 && sd.n_bwd_top > 0) {
     constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);   // staged inverse block
     // set once per process; a function-local static initialises thread-safely
@@ -740,10 +771,12 @@ void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, doubl
       return true;
     }();
     (void)attr;
-    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top, y, x);
+    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top * bt.count, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top,
+                                                                           y, x, bt);
   }
   if (part == 0 && sd.n_bwd > 0)
-    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, y, x);
+    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd * bt.count, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items,
+                                                                             y, x, bt);
   HIPCHK(hipGetLastError());
 }
 

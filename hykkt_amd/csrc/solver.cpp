@@ -1,9 +1,13 @@
-// solver.cpp -- HykktSolver::execute() on one MI355X.
+// solver.cpp -- HykktSolver::execute() on one MI355X, for one system or a
+// batch of systems that share one sparsity pattern.
 //
 // Step order follows HykktSolver.cpp:240-457 (+ check_error :846-930).
-// Everything per solve runs on one HIP stream; the host only launches and
-// reads back the CG convergence scalar (as the reference does) and the
-// final residual.
+// Everything per solve runs on one HIP stream.  Every per-system buffer of a
+// pattern lives in one arena, system k's copy `stride` doubles after system
+// k-1's, so each launch covers all the systems of the batch (system
+// blockIdx.y) and a system's arithmetic is the same alone or in a batch.
+// The host reads back the CG live flags between chunks of iterations and
+// the residuals once per pass.
 #include "solver.hpp"
 
 #include <algorithm>
@@ -15,6 +19,23 @@
 #include <numeric>
 
 namespace hykkt {
+
+// Per-system arena slots (offsets in doubles from the system's base).
+struct Solver::Slots {
+  int64_t in[IN_COUNT] = {}, gam = 0;
+  int64_t jd_vs = 0, ryd_s = 0, rx_til = 0, htil = 0, jcs = 0, ry_s = 0, max_d = 0, scale = 0;
+  int64_t jch_v = 0, jcth_v = 0, rxp = 0, tmp_n = 0, t1 = 0;
+  int64_t cx = 0, cr = 0, cw = 0, cp = 0, cs = 0, schur = 0, sc = 0, part = 0;
+  int64_t dx = 0, dsv = 0, dyc = 0, dyd = 0, sq_n = 0, sq_m = 0, sq_d2 = 0, sq_d4 = 0, res = 0;
+  int64_t e_x = 0, e_s = 0, e_y = 0, e_d = 0, e_rxt = 0, e_ysc = 0, e_rxp = 0, e_schur = 0;
+  int64_t e_cx = 0, c_dx = 0, c_dsv = 0, c_dyc = 0, c_dyd = 0;
+  int64_t L = 0, U = 0, dinv = 0, y = 0, ubuf = 0, bpart = 0;
+  int64_t stride = 0;
+};
+
+// per-system int flags, one array of `cap` each
+enum { FL_PIVOT = 0, FL_ZERO_ROW, FL_LIVE, FL_ACT, FL_KEEP, FL_CG_IT, FL_CG_ST, FL_COUNT };
+constexpr int RES_SLOTS = 8;
 
 struct Solver::Sym {
   int n = 0, m = 0, d = 0, nnz_h = 0, nnz_jc = 0, nnz_jd = 0, nnz_ds = 0;
@@ -33,18 +54,19 @@ struct Solver::Sym {
   DevBuf<int> hg_dsrc, hg_tp, hg_ta, hg_tb, ent_src;
   DevBuf<int> jch_rp, jch_ci, jch_map, jcth_rp, jcth_ci, jcth_map;
   DevBuf<int> phat_d, ipos_d;
-  DevBuf<double> h_v, jc_v, jd_v, ds_v, rx, rs, ry, ryd, y0;
-  DevBuf<double> jd_vs, ryd_s, rx_til, htil, jcs, ry_s, max_d, scale;
-  DevBuf<double> jch_v, jcth_v, bhat, rxp, tmp_n, t1;
-  DevBuf<double> cx, cr, cw, cp, cs, schur, sc, part;
-  DevBuf<double> dx, dsv, dyc, dyd, sq_n, sq_m, sq_d2, sq_d4, res;
-  // refinement: signed residual rows, scaled rhs, correction / candidate
-  DevBuf<double> e_x, e_s, e_y, e_d, e_rxt, e_ysc, e_rxp, e_schur, e_cx;
-  DevBuf<double> c_dx, c_dsv, c_dyc, c_dyd;
-  DevBuf<int> zero_row;   // smallest all-zero row seen by the Ruiz sweeps (INT_MAX: none)
-  // current value pointers (device)
-  const double *hv = nullptr, *jcv = nullptr, *jdv = nullptr, *dsd = nullptr;
-  const double *drx = nullptr, *drs = nullptr, *dry = nullptr, *dryd = nullptr;
+  // row blocks of the LDS-staged SpMVs: Jd^T, Jc^T (internal order), Jc
+  DevBuf<int> jdt_blk, jcth_blk, jch_blk;
+  int jdt_nblk = 0, jcth_nblk = 0, jch_nblk = 0;
+  // per-system storage for `cap` systems, and its flags and staging
+  Slots z;
+  int cap = 0;
+  DevBuf<double> arena;
+  DevBuf<int> flags;         // FL_COUNT arrays of cap ints
+  DevBuf<double> gath;       // cap * RES_SLOTS: per-system scalars gathered for a readback
+  DevBuf<double> gam_dev;    // cap gammas (staging for the copy into the arena)
+  DevBuf<const double*> tab; // cap * (IN_COUNT + 1) caller pointers for copy_in
+  std::vector<const double*> tab_h;
+  int* flag(int which) { return flags.p + static_cast<int64_t>(which) * cap; }
 };
 
 namespace {
@@ -56,13 +78,34 @@ void upload_or_dummy(DevBuf<T>& b, const std::vector<T>& v)
   else b.upload(v);
 }
 
+// Row blocks of the LDS-staged SpMV (see dev::spmv): up to 256 rows whose
+// entries fit the LDS stage together; a longer row is a block of its own.
+std::vector<int> spmv_blocks(const std::vector<int>& rp, int nrows)
+{
+  const int E = dev::spmv_lds_entries(), R = 256;
+  std::vector<int> b{0};
+  int r = 0;
+  while (r < nrows) {
+    const int e0 = rp[r];
+    int r1 = r + 1;
+    if (rp[r + 1] - e0 <= E)
+      while (r1 < nrows && r1 - r < R && rp[r1 + 1] - e0 <= E) r1++;
+    b.push_back(r1);
+    r = r1;
+  }
+  return b;
+}
+
 }  // namespace
 
-Solver::Solver(double gamma) : gamma_(gamma)
+Solver::Solver(double gamma, int capacity) : gamma_(gamma), cap_(capacity)
 {
+  HYKKT_REQUIRE(capacity >= 1, HYKKT_E_ARG, "batch capacity must be at least 1");
   hykkt_default_options(&opts);
   HIPCHK(hipStreamCreate(&st_));
   for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
+  sys_in_.resize(cap_);
+  st_k_.resize(cap_);
 }
 
 Solver::~Solver()
@@ -72,6 +115,8 @@ Solver::~Solver()
     if (e) (void)hipEventDestroy(e);
   if (st_) (void)hipStreamDestroy(st_);
 }
+
+double* Solver::P(int64_t off) const { return sym_->arena.p + off; }
 
 void Solver::set_block(int which, const int* rp, const int* ci, const double* v, int n, int m,
                        int nnz)
@@ -108,6 +153,18 @@ void Solver::set_perm(const int* perm, int n)
   }
   user_perm_ = std::move(p);
   sym_.reset();   // re-analyse with the new ordering
+}
+
+void Solver::bind_system(int k, double gamma, const double* const vals[IN_COUNT])
+{
+  HYKKT_REQUIRE(k >= 0 && k < cap_, HYKKT_E_ARG, "system index beyond the batch capacity");
+  for (int j = 0; j < IN_COUNT; j++)
+    HYKKT_REQUIRE(vals[j] || j == IN_Y0 || j == IN_JD, HYKKT_E_ARG,
+                  "null value array for a batch system");
+  SystemIn& s = sys_in_[k];
+  for (int j = 0; j < IN_COUNT; j++) s.v[j] = vals[j];
+  s.gamma = gamma;
+  s.set = true;
 }
 
 void Solver::read_files(const char* const files[8], int skip)
@@ -287,127 +344,177 @@ void Solver::analyze()
   upload_or_dummy(S->jcth_map, jcth_map);
   S->phat_d.upload(S->phat);
   S->ipos_d.upload(S->ipos);
+  {
+    std::vector<int> b = spmv_blocks(S->jdt.at.rp, n);
+    S->jdt_nblk = static_cast<int>(b.size()) - 1;
+    S->jdt_blk.upload(b);
+    b = spmv_blocks(jcht.at.rp, n);
+    S->jcth_nblk = static_cast<int>(b.size()) - 1;
+    S->jcth_blk.upload(b);
+    b = spmv_blocks(jch.rp, m);
+    S->jch_nblk = static_cast<int>(b.size()) - 1;
+    S->jch_blk.upload(b);
+  }
 
-  // ---- device work buffers ----
-  auto A = [](DevBuf<double>& b, size_t k) { b.alloc(std::max<size_t>(k, 1)); };
-  A(S->jd_vs, S->nnz_jd);
-  A(S->ryd_s, d);
-  A(S->rx_til, n);
-  A(S->htil, S->htp.e.nnz());
-  A(S->jcs, S->nnz_jc);
-  A(S->ry_s, m);
-  A(S->max_d, n + m);
-  A(S->scale, n + m);
-  A(S->jch_v, S->nnz_jc);
-  A(S->jcth_v, S->nnz_jc);
-  A(S->bhat, n);
-  A(S->rxp, n);
-  A(S->tmp_n, n);
-  A(S->t1, n);
-  for (DevBuf<double>* b : {&S->cx, &S->cr, &S->cw, &S->cp, &S->cs, &S->schur}) A(*b, m);
-  A(S->sc, 8);
-  A(S->part, dev::reduce_partials());
-  A(S->dx, n);
-  A(S->dsv, d);
-  A(S->dyc, m);
-  A(S->dyd, d);
-  A(S->sq_n, n);
-  A(S->sq_m, m);
-  A(S->sq_d2, d);
-  A(S->sq_d4, d);
-  A(S->res, 16);
-  for (DevBuf<double>* b : {&S->e_x, &S->e_rxt, &S->e_rxp, &S->c_dx}) A(*b, n);
-  for (DevBuf<double>* b : {&S->e_y, &S->e_ysc, &S->e_schur, &S->e_cx, &S->c_dyc}) A(*b, m);
-  for (DevBuf<double>* b : {&S->e_s, &S->e_d, &S->c_dsv, &S->c_dyd}) A(*b, d);
-  A(S->y0, m);
-  S->zero_row.alloc(1);
-  HIPCHK(hipDeviceSynchronize());
+  // ---- the per-system arena: slots in 256-byte units ----
+  Slots& z = S->z;
+  int64_t at = 0;
+  auto slot = [&](int64_t len) {
+    const int64_t o = at;
+    at += (std::max<int64_t>(len, 1) + 31) / 32 * 32;
+    return o;
+  };
+  z.in[IN_H] = slot(S->nnz_h);
+  z.in[IN_DS] = slot(d);
+  z.in[IN_JC] = slot(S->nnz_jc);
+  z.in[IN_JD] = slot(S->nnz_jd);
+  z.in[IN_RX] = slot(n);
+  z.in[IN_RS] = slot(d);
+  z.in[IN_RY] = slot(m);
+  z.in[IN_RYD] = slot(d);
+  z.in[IN_Y0] = slot(m);
+  z.gam = slot(1);
+  z.jd_vs = slot(S->nnz_jd);
+  z.ryd_s = slot(d);
+  z.rx_til = slot(n);
+  z.htil = slot(S->htp.e.nnz());
+  z.jcs = slot(S->nnz_jc);
+  z.ry_s = slot(m);
+  z.max_d = slot(n + m);
+  z.scale = slot(n + m);
+  z.jch_v = slot(S->nnz_jc);
+  z.jcth_v = slot(S->nnz_jc);
+  z.rxp = slot(n);
+  z.tmp_n = slot(n);
+  z.t1 = slot(n);
+  z.cx = slot(m);
+  z.cr = slot(m);
+  z.cw = slot(m);
+  z.cp = slot(m);
+  z.cs = slot(m);
+  z.schur = slot(m);
+  z.sc = slot(8);
+  z.part = slot(std::max(256, schur_partials(m, S->jch_nblk)));
+  z.dx = slot(n);
+  z.dsv = slot(d);
+  z.dyc = slot(m);
+  z.dyd = slot(d);
+  z.sq_n = slot(n);
+  z.sq_m = slot(m);
+  z.sq_d2 = slot(d);
+  z.sq_d4 = slot(d);
+  z.res = slot(RES_SLOTS);
+  z.e_x = slot(n);
+  z.e_s = slot(d);
+  z.e_y = slot(m);
+  z.e_d = slot(d);
+  z.e_rxt = slot(n);
+  z.e_ysc = slot(m);
+  z.e_rxp = slot(n);
+  z.e_schur = slot(m);
+  z.e_cx = slot(m);
+  z.c_dx = slot(n);
+  z.c_dsv = slot(d);
+  z.c_dyc = slot(m);
+  z.c_dyd = slot(d);
+  const CholEngine::Sizes cz = S->chol->sizes();
+  z.L = slot(cz.L);
+  z.U = slot(cz.U);
+  z.dinv = slot(cz.dinv);
+  z.y = slot(cz.y);
+  z.ubuf = slot(cz.ubuf);
+  z.bpart = slot(cz.bpart);
+  z.stride = at;
+  S->cap = cap_;
+  S->arena.alloc(static_cast<size_t>(z.stride) * cap_);
+  S->flags.alloc(static_cast<size_t>(FL_COUNT) * cap_);
+  S->gath.alloc(static_cast<size_t>(RES_SLOTS) * cap_);
+  S->gam_dev.alloc(static_cast<size_t>(cap_));
+  S->tab.alloc(static_cast<size_t>(IN_COUNT + 1) * cap_);
+  double* base = S->arena.p;
+  S->chol->bind(CholEngine::Storage{base + z.L, base + z.U, base + z.dinv, base + z.y,
+                                    base + z.ubuf, base + z.bpart, S->flags.p + FL_PIVOT * cap_},
+                z.stride, cap_);
+  HIPCHK(hipStreamSynchronize(st_));
 
-  stats.n = n;
-  stats.m = m;
-  stats.d = d;
-  stats.nnz_htil = S->htp.e.nnz();
-  stats.nnz_hgam = HG.nnz();
-  stats.nnz_l = pl.nnz_l_stored;
-  stats.n_supernodes = pl.ns;
-  stats.n_levels = static_cast<int>(pl.level_ptr.size()) - 1;
-  stats.max_front = pl.max_front;
-  stats.factor_flops = pl.flops;
-  stats.t_symbolic_ms =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (hykkt_stats& st : st_k_) {
+    st.n = n;
+    st.m = m;
+    st.d = d;
+    st.nnz_htil = S->htp.e.nnz();
+    st.nnz_hgam = HG.nnz();
+    st.nnz_l = pl.nnz_l_stored;
+    st.n_supernodes = pl.ns;
+    st.n_levels = static_cast<int>(pl.level_ptr.size()) - 1;
+    st.max_front = pl.max_front;
+    st.factor_flops = pl.flops;
+    st.t_symbolic_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  last_iters_ = 0;
   sym_ = std::move(S);
 }
 
 // ------------------------------------------------------------------ values
-const double* Solver::vec(int which) const { return vecs_[which]; }
-
-void Solver::upload_values()
+// Copy each system's values into its arena slots: one copy kernel over all
+// systems from device pointers, or host-to-device copies from host arrays.
+void Solver::upload_values(int count)
 {
   Sym& S = *sym_;
-  auto need = [&](int which, int len, const char* name) {
-    HYKKT_REQUIRE(vecs_[which] != nullptr, HYKKT_E_STATE, std::string(name) + " not set");
-    HYKKT_REQUIRE(vec_n_[which] == 0 || vec_n_[which] >= len, HYKKT_E_ARG,
-                  std::string(name) + " too short");
-  };
-  need(VEC_RX, S.n, "rx");
-  need(VEC_RS, S.d, "rs");
-  need(VEC_RY, S.m, "ry");
-  need(VEC_RYD, S.d, "ryd");
+  const Slots& z = S.z;
+  const dev::Batch b{count, z.stride, nullptr};
+  const int64_t len[IN_COUNT + 1] = {S.nnz_h, S.d, S.nnz_jc, S.nnz_jd, S.n, S.d, S.m, S.d, S.m, 1};
+  bool any_y0 = false;
+  std::vector<double> gam(count);
+  for (int k = 0; k < count; k++) {
+    gam[k] = sys_in_[k].gamma;
+    any_y0 = any_y0 || sys_in_[k].v[IN_Y0] != nullptr;
+  }
+  // a system without its own start y begins at zero
+  if (any_y0) dev::fill(P(z.in[IN_Y0]), 0.0, S.m, b, st_);
   if (opts.inputs_on_device) {
-    S.hv = blk_[BLK_H].v;
-    S.jcv = blk_[BLK_JC].v;
-    S.jdv = blk_[BLK_JD].v;
-    S.dsd = blk_[BLK_DS].v;
-    S.drx = vecs_[VEC_RX];
-    S.drs = vecs_[VEC_RS];
-    S.dry = vecs_[VEC_RY];
-    S.dryd = vecs_[VEC_RYD];
-    if (vecs_[VEC_Y]) dev::copy(S.y0.p, vecs_[VEC_Y], S.m, st_);
-    else dev::fill(S.y0.p, 0.0, S.m, st_);
+    HIPCHK(hipMemcpyAsync(S.gam_dev.p, gam.data(), sizeof(double) * count, hipMemcpyHostToDevice,
+                          st_));
+    S.tab_h.assign(static_cast<size_t>(IN_COUNT + 1) * count, nullptr);
+    for (int k = 0; k < count; k++) {
+      for (int j = 0; j < IN_COUNT; j++) S.tab_h[k * (IN_COUNT + 1) + j] = sys_in_[k].v[j];
+      S.tab_h[k * (IN_COUNT + 1) + IN_COUNT] = S.gam_dev.p + k;
+    }
+    HIPCHK(hipMemcpyAsync(S.tab.p, S.tab_h.data(), sizeof(double*) * S.tab_h.size(),
+                          hipMemcpyHostToDevice, st_));
+    double* dst[IN_COUNT + 1];
+    for (int j = 0; j < IN_COUNT; j++) dst[j] = P(z.in[j]);
+    dst[IN_COUNT] = P(z.gam);
+    dev::copy_in(IN_COUNT + 1, dst, S.tab.p, len, b, st_);
+    // gam and the pointer table are host memory the copies read: wait for them
+    HIPCHK(hipStreamSynchronize(st_));
   } else {
-    // value-only H2D per solve (HykktSolver.cpp:635-666), into staging
-    // buffers made on first use (the option inputs_on_device may change
-    // between solves without a new analysis)
-    auto h2d = [&](DevBuf<double>& b, const double* src, size_t cnt) {
-      b.ensure(std::max<size_t>(cnt, 1));
-      if (cnt) HIPCHK(hipMemcpyAsync(b.p, src, cnt * sizeof(double), hipMemcpyHostToDevice, st_));
-    };
-    h2d(S.h_v, blk_[BLK_H].v, S.nnz_h);
-    h2d(S.jc_v, blk_[BLK_JC].v, S.nnz_jc);
-    h2d(S.jd_v, blk_[BLK_JD].v, S.nnz_jd);
-    h2d(S.ds_v, blk_[BLK_DS].v, S.d);
-    h2d(S.rx, vecs_[VEC_RX], S.n);
-    h2d(S.rs, vecs_[VEC_RS], S.d);
-    h2d(S.ry, vecs_[VEC_RY], S.m);
-    h2d(S.ryd, vecs_[VEC_RYD], S.d);
-    if (vecs_[VEC_Y]) HIPCHK(hipMemcpyAsync(S.y0.p, vecs_[VEC_Y], S.m * sizeof(double),
-                                            hipMemcpyHostToDevice, st_));
-    else dev::fill(S.y0.p, 0.0, S.m, st_);
-    S.hv = S.h_v.p;
-    S.jcv = S.jc_v.p;
-    S.jdv = S.jd_v.p;
-    S.dsd = S.ds_v.p;
-    S.drx = S.rx.p;
-    S.drs = S.rs.p;
-    S.dry = S.ry.p;
-    S.dryd = S.ryd.p;
+    for (int k = 0; k < count; k++) {
+      double* base = P(static_cast<int64_t>(k) * z.stride);
+      for (int j = 0; j < IN_COUNT; j++)
+        if (sys_in_[k].v[j] && len[j])
+          HIPCHK(hipMemcpyAsync(base + z.in[j], sys_in_[k].v[j], sizeof(double) * len[j],
+                                hipMemcpyHostToDevice, st_));
+      HIPCHK(hipMemcpyAsync(base + z.gam, &gam[k], sizeof(double), hipMemcpyHostToDevice, st_));
+    }
+    HIPCHK(hipStreamSynchronize(st_));   // gam is read by the copies
   }
 }
-
-void Solver::chol_solve(double* x) { sym_->chol->solve(x); }
 
 // compute_spgemm_hgamma + apply_permutation + add_diag (HykktSolver.cpp:370-395):
 // the lower triangle of P*Hg*P^T, each value evaluated from the Ruiz-scaled
 // H~ and Jc and written straight to its slot in the factor storage, then the
 // diagonal shift.
-void Solver::assemble_hgamma()
+void Solver::assemble_hgamma(const dev::Batch& b)
 {
   Sym& S = *sym_;
+  const Slots& z = S.z;
   CholEngine& ch = *S.chol;
-  ch.zero_L();
-  dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, S.htil.p,
-                      S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, S.jcs.p, S.jcs.p, gamma_, ch.L(), st_);
-  ch.shift_diag(opts.delta);   // add_diag(delta) hook (HykktSolver.cpp:392)
+  ch.zero_L(b);
+  dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, P(z.htil),
+                      S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, P(z.jcs), P(z.jcs), P(z.gam), ch.L(), b,
+                      st_);
+  ch.shift_diag(opts.delta, b);   // add_diag(delta) hook (HykktSolver.cpp:392)
 }
 
 // ------------------------------------------------------------------ execute
@@ -418,252 +525,379 @@ int Solver::execute()
     if (opts.verbose) std::printf("\n\nERROR: USING HYKKT WITH NEW NONZERO STRUCTURE\n\n");
     return HYKKT_RESIDUAL_FAIL;
   }
+  const std::vector<int> rc = run(1, true);
+  stats = st_k_[0];
+  if (opts.verbose) {
+    std::printf("||Ax-b||/||b|| = %32.32g\n\n", stats.resid2);
+    std::printf(stats.resid2 < opts.norm_tol ? "Residual test passed\n" : "Residual test failed\n");
+  }
+  return rc[0];
+}
+
+std::vector<int> Solver::execute_batch(int count)
+{
+  HYKKT_REQUIRE(count >= 1 && count <= cap_, HYKKT_E_ARG, "batch count outside 1 .. capacity");
+  for (int k = 0; k < count; k++)
+    HYKKT_REQUIRE(sys_in_[k].set, HYKKT_E_STATE,
+                  "batch system " + std::to_string(k) + " has no bound values");
+  std::vector<int> rc = run(count, false);
+  stats = st_k_[0];
+  return rc;
+}
+
+const hykkt_stats& Solver::stats_of(int k) const
+{
+  HYKKT_REQUIRE(k >= 0 && k < cap_, HYKKT_E_ARG, "system index beyond the batch capacity");
+  return st_k_[k];
+}
+
+std::vector<int> Solver::run(int count, bool from_setters)
+{
   for (int k = 0; k < 4; k++)
     HYKKT_REQUIRE(blk_[k].set, HYKKT_E_STATE, "a matrix block was not set");
+  if (from_setters) {
+    auto need = [&](int which, int len, const char* name) {
+      HYKKT_REQUIRE(vecs_[which] != nullptr, HYKKT_E_STATE, std::string(name) + " not set");
+      HYKKT_REQUIRE(vec_n_[which] == 0 || vec_n_[which] >= len, HYKKT_E_ARG,
+                    std::string(name) + " too short");
+    };
+    need(VEC_RX, blk_[BLK_H].n, "rx");
+    need(VEC_RS, blk_[BLK_DS].n, "rs");
+    need(VEC_RY, blk_[BLK_JC].n, "ry");
+    need(VEC_RYD, blk_[BLK_DS].n, "ryd");
+    SystemIn& s0 = sys_in_[0];
+    s0.v[IN_H] = blk_[BLK_H].v;
+    s0.v[IN_DS] = blk_[BLK_DS].v;
+    s0.v[IN_JC] = blk_[BLK_JC].v;
+    s0.v[IN_JD] = blk_[BLK_JD].v;
+    s0.v[IN_RX] = vecs_[VEC_RX];
+    s0.v[IN_RS] = vecs_[VEC_RS];
+    s0.v[IN_RY] = vecs_[VEC_RY];
+    s0.v[IN_RYD] = vecs_[VEC_RYD];
+    s0.v[IN_Y0] = vecs_[VEC_Y];
+    s0.gamma = gamma_;
+    s0.set = false;   // the batch binding of system 0 is not kept
+  }
   if (!sym_ || !pattern_matches() || sym_->has_jd != (blk_[BLK_JD].nnz > 0)) analyze();
   Sym& S = *sym_;
+  const Slots& z = S.z;
   const int n = S.n, m = S.m, d = S.d;
-  const double gam = gamma_;
+  const dev::Batch b{count, z.stride, nullptr};
   HIPCHK(hipEventRecord(ev_[0], st_));
-  upload_values();
+  upload_values(count);
+  bool any_y0 = false;
+  for (int k = 0; k < count; k++) any_y0 = any_y0 || sys_in_[k].v[IN_Y0] != nullptr;
+  const double* in[IN_COUNT];
+  for (int j = 0; j < IN_COUNT; j++) in[j] = P(z.in[j]);
 
   // ---- compute_spgemm_htil (HykktSolver.cpp:289-337) ----
   if (S.has_jd) {
-    dev::row_scale(d, S.jd_rp.p, S.jdv, S.dsd, S.dryd, S.drs, S.jd_vs.p, S.ryd_s.p, st_);
-    dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, S.jdv, 1.0, S.ryd_s.p, 1.0, S.drx,
-              S.rx_til.p, st_);
-    dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, S.hv, S.ht_tp.p, S.ht_ta.p, S.ht_tb.p, S.jdv,
-                       S.jd_vs.p, 1.0, S.htil.p, st_);
+    dev::row_scale(d, S.jd_rp.p, in[IN_JD], in[IN_DS], in[IN_RYD], in[IN_RS], P(z.jd_vs),
+                   P(z.ryd_s), b, st_);
+    dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s), 1.0,
+              in[IN_RX], P(z.rx_til), S.jdt_blk.p, S.jdt_nblk, b, st_);
+    dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, in[IN_H], S.ht_tp.p, S.ht_ta.p, S.ht_tb.p,
+                       in[IN_JD], P(z.jd_vs), 1.0, P(z.htil), b, st_);
   } else {
-    dev::copy(S.rx_til.p, S.drx, n, st_);
-    dev::copy(S.htil.p, S.hv, S.nnz_h, st_);
+    dev::copy(P(z.rx_til), in[IN_RX], n, b, st_);
+    dev::copy(P(z.htil), in[IN_H], S.nnz_h, b, st_);
   }
-  dev::copy(S.jcs.p, S.jcv, S.nnz_jc, st_);
-  dev::copy(S.ry_s.p, S.dry, m, st_);
+  dev::copy(P(z.jcs), in[IN_JC], S.nnz_jc, b, st_);
+  dev::copy(P(z.ry_s), in[IN_RY], m, b, st_);
 
   // ---- compute_ruiz_scaling (HykktSolver.cpp:340-349; RuizClass.cpp:54-98) ----
-  if (opts.ruiz_its <= 0) dev::fill(S.max_d.p, 1.0, n + m, st_);
-  dev::fill_i(S.zero_row.p, INT_MAX, 1, st_);
+  if (opts.ruiz_its <= 0) dev::fill(P(z.max_d), 1.0, n + m, b, st_);
+  dev::fill_i(S.flag(FL_ZERO_ROW), INT_MAX, count, st_);
   for (int it = 0; it < opts.ruiz_its; it++) {
-    dev::ruiz_rowmax(n, n + m, S.ht_rp.p, S.htil.p, S.jct_rp.p, S.jct_map.p, S.jcs.p, S.jc_rp.p,
-                     S.jcs.p, S.scale.p, st_, S.zero_row.p);
-    dev::ruiz_scale(n, n + m, S.ht_rp.p, S.ht_ci.p, S.htil.p, S.jct_rp.p, S.jct_ci.p, nullptr,
-                    S.jc_rp.p, S.jc_ci.p, S.jcs.p, S.scale.p, S.rx_til.p, S.ry_s.p, S.max_d.p,
-                    it == 0, st_);
+    dev::ruiz_rowmax(n, n + m, S.ht_rp.p, P(z.htil), S.jct_rp.p, S.jct_map.p, P(z.jcs), S.jc_rp.p,
+                     P(z.jcs), P(z.scale), S.flag(FL_ZERO_ROW), b, st_);
+    dev::ruiz_scale(n, n + m, S.ht_rp.p, S.ht_ci.p, P(z.htil), S.jct_rp.p, S.jct_ci.p, nullptr,
+                    S.jc_rp.p, S.jc_ci.p, P(z.jcs), P(z.scale), P(z.rx_til), P(z.ry_s),
+                    P(z.max_d), it == 0, b, st_);
   }
 
   // ---- H_gamma straight into the factor storage (HykktSolver.cpp:370-395) ----
   CholEngine& ch = *S.chol;
-  assemble_hgamma();
+  assemble_hgamma(b);
 
   // ---- compute_hgamma_factorization (HykktSolver.cpp:398-401) ----
-  ch.factor(opts.pivot_tol);
+  ch.factor(opts.pivot_tol, b);
   HIPCHK(hipEventRecord(ev_[1], st_));
 
   // permuted Jc and Jc^T values (apply_permutation, HykktSolver.cpp:386-395)
-  dev::gather(S.jch_v.p, S.jcs.p, S.jch_map.p, S.nnz_jc, st_);
-  dev::gather(S.jcth_v.p, S.jcs.p, S.jcth_map.p, S.nnz_jc, st_);
+  dev::gather(P(z.jch_v), P(z.jcs), S.jch_map.p, S.nnz_jc, b, st_);
+  dev::gather(P(z.jcth_v), P(z.jcs), S.jcth_map.p, S.nnz_jc, b, st_);
 
   // ---- first pass: the reference's solve (HykktSolver.cpp:404-457) ----
-  Pass first{S.rxp.p, S.schur.p, S.cx.p};
-  int iters = 0, cg_status = 0;
-  double fnorm = 0.0;
   // CG start y_host, or zero when none is bound (then no start-residual solve)
-  solve_phase(first, S.rx_til.p, S.ry_s.p, S.drs, S.dryd, vecs_[VEC_Y] ? S.y0.p : nullptr, S.dx.p,
-              S.dsv.p, S.dyc.p, S.dyd.p, &iters, &fnorm, &cg_status, ev_[2]);
+  const std::vector<CgResult> cg = solve_phase(Pass{z.rxp, z.schur, z.cx}, z.rx_til, z.ry_s,
+                                               z.in[IN_RS], z.in[IN_RYD], any_y0, z.dx, z.dsv,
+                                               z.dyc, z.dyd, b, ev_[2]);
+  int it_max = 0;
+  for (const CgResult& r : cg) it_max = std::max(it_max, r.iters);
+  last_iters_ = it_max;
 
   // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
-  double hr[8];
-  int zrow = INT_MAX;
-  HIPCHK(hipMemcpyAsync(&zrow, S.zero_row.p, sizeof(int), hipMemcpyDeviceToHost, st_));
-  residual(S.dx.p, S.dsv.p, S.dyc.p, S.dyd.p, true, ev_[3], hr);
-  // matrix_vector_ops.cu:258-264 scales such a row by 1/sqrt(0) = inf and the
-  // solve turns to NaN; here it is an argument error
-  if (zrow != INT_MAX)
-    throw Error(HYKKT_E_ARG, zrow < n ? "row " + std::to_string(zrow) +
-                                            " of [H~ Jc^T] has no nonzero value (Ruiz scale 1/sqrt(0))"
-                                      : "row " + std::to_string(zrow - n) +
-                                            " of Jc has no nonzero value (Ruiz scale 1/sqrt(0))");
-  const int hpiv = ch.pivot();
+  residual(z.dx, z.dsv, z.dyc, z.dyd, true, b);
+  HIPCHK(hipEventRecord(ev_[3], st_));
+  std::vector<int> zrow(count), keep(count), act(count);
+  HIPCHK(hipMemcpyAsync(zrow.data(), S.flag(FL_ZERO_ROW), sizeof(int) * count,
+                        hipMemcpyDeviceToHost, st_));
+  std::vector<double> hr = read_res(count);
+  const std::vector<int> piv = ch.pivots(count);
   HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
                 "triangular-solve sweep exceeded its wait bound (results invalid)");
-  const double nb = hr[4] + hr[5] + hr[6] + hr[7];
-  stats.resid2 = std::sqrt(hr[0] + hr[1]) / std::sqrt(nb);
-  stats.resid4 = std::sqrt(hr[0] + hr[1] + hr[2] + hr[3]) / std::sqrt(nb);
-  stats.resid4_first = stats.resid4;
-  stats.refine_steps = 0;
+  std::vector<int> rc(count, HYKKT_OK);
+  std::vector<double> nb(count);
+  for (int k = 0; k < count; k++) {
+    hykkt_stats& st = st_k_[k];
+    const double* h = hr.data() + static_cast<size_t>(k) * RES_SLOTS;
+    nb[k] = h[4] + h[5] + h[6] + h[7];
+    st.resid2 = std::sqrt(h[0] + h[1]) / std::sqrt(nb[k]);
+    st.resid4 = std::sqrt(h[0] + h[1] + h[2] + h[3]) / std::sqrt(nb[k]);
+    st.resid4_first = st.resid4;
+    st.refine_steps = 0;
+    st.cg_iters = cg[k].iters;
+    st.cg_status = cg[k].status;
+    st.cg_final_norm = cg[k].final_norm;
+    st.pivot = piv[k];
+    // matrix_vector_ops.cu:258-264 scales a zero row by 1/sqrt(0) = inf and
+    // the solve turns to NaN; here it is an argument error of that system
+    if (zrow[k] != INT_MAX) rc[k] = HYKKT_E_ARG;
+  }
 
   // ---- iterative refinement on the 4x4 block KKT system (an addition: the
   // reference stops after one pass).  The signed residual rows go through the
   // right-hand-side half of formation (r~x = ex + Jd^T (Ds ed + es), then the
   // Ruiz scaling max_d) and the same solve phase with a zero CG start; a
-  // correction is kept only if it lowers resid4. ----
-  Pass corr{S.e_rxp.p, S.e_schur.p, S.e_cx.p};
-  for (int k = 0; hpiv < 0 && k < opts.refine_max && stats.resid4 > opts.refine_tol; k++) {
-    if (S.has_jd) {
-      dev::row_scale(d, S.jd_rp.p, S.jdv, S.dsd, S.e_d.p, S.e_s.p, nullptr, S.ryd_s.p, st_);
-      dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, S.jdv, 1.0, S.ryd_s.p, 1.0, S.e_x.p,
-                S.e_rxt.p, st_);
-    } else {
-      dev::copy(S.e_rxt.p, S.e_x.p, n, st_);
+  // correction is kept only if it lowers resid4.  Each system decides for
+  // itself, as it would alone; the pass covers the systems still refining. ----
+  std::vector<int> stopped(count, 0);
+  const Pass corr{z.e_rxp, z.e_schur, z.e_cx};
+  for (int pass = 0; pass < opts.refine_max; pass++) {
+    int nact = 0;
+    for (int k = 0; k < count; k++) {
+      act[k] = (rc[k] == HYKKT_OK && piv[k] < 0 && !stopped[k] &&
+                st_k_[k].resid4 > opts.refine_tol) ? 1 : 0;
+      nact += act[k];
     }
-    dev::vec_scale(S.e_rxt.p, S.max_d.p, n, st_);
-    dev::copy(S.e_ysc.p, S.e_y.p, m, st_);
-    dev::vec_scale(S.e_ysc.p, S.max_d.p + n, m, st_);
-    int it = 0, cgs = 0;
-    double fn = 0.0;
-    solve_phase(corr, S.e_rxt.p, S.e_ysc.p, S.e_s.p, S.e_d.p, nullptr, S.c_dx.p, S.c_dsv.p,
-                S.c_dyc.p, S.c_dyd.p, &it, &fn, &cgs, nullptr);
-    dev::add(S.c_dx.p, S.c_dx.p, S.dx.p, n, st_);
-    dev::add(S.c_dsv.p, S.c_dsv.p, S.dsv.p, d, st_);
-    dev::add(S.c_dyc.p, S.c_dyc.p, S.dyc.p, m, st_);
-    dev::add(S.c_dyd.p, S.c_dyd.p, S.dyd.p, d, st_);
-    double cr[8];
-    residual(S.c_dx.p, S.c_dsv.p, S.c_dyc.p, S.c_dyd.p, true, nullptr, cr);
+    if (nact == 0) break;
+    HIPCHK(hipMemcpyAsync(S.flag(FL_ACT), act.data(), sizeof(int) * count, hipMemcpyHostToDevice,
+                          st_));
+    const dev::Batch ba = dev::masked(b, S.flag(FL_ACT));
+    if (S.has_jd) {
+      dev::row_scale(d, S.jd_rp.p, in[IN_JD], in[IN_DS], P(z.e_d), P(z.e_s), nullptr, P(z.ryd_s),
+                     ba, st_);
+      dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s), 1.0,
+                P(z.e_x), P(z.e_rxt), S.jdt_blk.p, S.jdt_nblk, ba, st_);
+    } else {
+      dev::copy(P(z.e_rxt), P(z.e_x), n, ba, st_);
+    }
+    dev::vec_scale(P(z.e_rxt), P(z.max_d), n, ba, st_);
+    dev::copy(P(z.e_ysc), P(z.e_y), m, ba, st_);
+    dev::vec_scale(P(z.e_ysc), P(z.max_d) + n, m, ba, st_);
+    solve_phase(corr, z.e_rxt, z.e_ysc, z.e_s, z.e_d, false, z.c_dx, z.c_dsv, z.c_dyc, z.c_dyd, ba,
+                nullptr);
+    dev::add(P(z.c_dx), P(z.c_dx), P(z.dx), n, ba, st_);
+    dev::add(P(z.c_dsv), P(z.c_dsv), P(z.dsv), d, ba, st_);
+    dev::add(P(z.c_dyc), P(z.c_dyc), P(z.dyc), m, ba, st_);
+    dev::add(P(z.c_dyd), P(z.c_dyd), P(z.dyd), d, ba, st_);
+    residual(z.c_dx, z.c_dsv, z.c_dyc, z.c_dyd, true, ba);
+    hr = read_res(count);
     HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
                   "triangular-solve sweep exceeded its wait bound (results invalid)");
-    const double r4 = std::sqrt(cr[0] + cr[1] + cr[2] + cr[3]) / std::sqrt(nb);
-    if (!(r4 < stats.resid4)) break;   // keep the better solution; the rows in e_* are stale
-    S.dx.swap(S.c_dx);
-    S.dsv.swap(S.c_dsv);
-    S.dyc.swap(S.c_dyc);
-    S.dyd.swap(S.c_dyd);
-    stats.resid2 = std::sqrt(cr[0] + cr[1]) / std::sqrt(nb);
-    stats.resid4 = r4;
-    stats.refine_steps++;
+    int nkeep = 0;
+    for (int k = 0; k < count; k++) {
+      keep[k] = 0;
+      if (!act[k]) continue;
+      const double* h = hr.data() + static_cast<size_t>(k) * RES_SLOTS;
+      const double c4 = std::sqrt(h[0] + h[1] + h[2] + h[3]) / std::sqrt(nb[k]);
+      if (!(c4 < st_k_[k].resid4)) {   // keep the better solution; its rows in e_* are stale
+        stopped[k] = 1;
+        continue;
+      }
+      keep[k] = 1;
+      nkeep++;
+      st_k_[k].resid2 = std::sqrt(h[0] + h[1]) / std::sqrt(nb[k]);
+      st_k_[k].resid4 = c4;
+      st_k_[k].refine_steps++;
+    }
+    if (nkeep == 0) break;
+    HIPCHK(hipMemcpyAsync(S.flag(FL_KEEP), keep.data(), sizeof(int) * count,
+                          hipMemcpyHostToDevice, st_));
+    dev::copy_where(P(z.dx), P(z.c_dx), n, S.flag(FL_KEEP), b, st_);
+    dev::copy_where(P(z.dsv), P(z.c_dsv), d, S.flag(FL_KEEP), b, st_);
+    dev::copy_where(P(z.dyc), P(z.c_dyc), m, S.flag(FL_KEEP), b, st_);
+    dev::copy_where(P(z.dyd), P(z.c_dyd), d, S.flag(FL_KEEP), b, st_);
   }
   HIPCHK(hipEventRecord(ev_[4], st_));
   HIPCHK(hipEventSynchronize(ev_[4]));
   prof.collect();
 
-  stats.cg_iters = iters;
-  stats.cg_status = cg_status;
-  stats.cg_final_norm = fnorm;
-  stats.pivot = hpiv;
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[4]));
-  stats.t_solve_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-  stats.t_factor_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
-  stats.t_cg_ms = ms;
-  HIPCHK(hipEventElapsedTime(&ms, ev_[3], ev_[4]));
-  stats.t_refine_ms = ms;
-  stats.n_solves++;
-  if (opts.verbose) {
-    std::printf("||Ax-b||/||b|| = %32.32g\n\n", stats.resid2);
-    std::printf(stats.resid2 < opts.norm_tol ? "Residual test passed\n" : "Residual test failed\n");
+  float ms[4] = {};
+  HIPCHK(hipEventElapsedTime(&ms[0], ev_[0], ev_[4]));
+  HIPCHK(hipEventElapsedTime(&ms[1], ev_[0], ev_[1]));
+  HIPCHK(hipEventElapsedTime(&ms[2], ev_[1], ev_[2]));
+  HIPCHK(hipEventElapsedTime(&ms[3], ev_[3], ev_[4]));
+  for (int k = 0; k < count; k++) {
+    hykkt_stats& st = st_k_[k];
+    st.t_solve_ms = ms[0];
+    st.t_factor_ms = ms[1];
+    st.t_cg_ms = ms[2];
+    st.t_refine_ms = ms[3];
+    st.n_solves++;
+    if (rc[k] != HYKKT_OK) continue;
+    if (st.pivot >= 0) rc[k] = HYKKT_RESIDUAL_FAIL;
+    else rc[k] = (st.resid2 < opts.norm_tol) ? HYKKT_OK : HYKKT_RESIDUAL_FAIL;
   }
-  if (hpiv >= 0) return HYKKT_RESIDUAL_FAIL;
-  return (stats.resid2 < opts.norm_tol) ? HYKKT_OK : HYKKT_RESIDUAL_FAIL;
+  if (from_setters && zrow[0] != INT_MAX)
+    throw Error(HYKKT_E_ARG, zrow[0] < n ? "row " + std::to_string(zrow[0]) +
+                                              " of [H~ Jc^T] has no nonzero value (Ruiz scale 1/sqrt(0))"
+                                        : "row " + std::to_string(zrow[0] - n) +
+                                              " of Jc has no nonzero value (Ruiz scale 1/sqrt(0))");
+  return rc;
 }
 
 // setup_conjugate_gradient (HykktSolver.cpp:814-843), the Schur CG
 // (SchurComplementConjugateGradient.cpp:89-194) and recover_solution
 // (HykktSolver.cpp:410-457) for Ruiz-scaled right-hand sides rxt (n, natural
 // order) and ysc (m); rs, ryd are the unscaled d-rows used by the recovery.
-void Solver::solve_phase(const Pass& w, const double* rxt, const double* ysc, const double* rs,
-                         const double* ryd, const double* x0, double* dx, double* dsv,
-                         double* dyc, double* dyd, int* iters, double* fnorm, int* cg_status,
-                         hipEvent_t after_cg)
+// All arguments are arena offsets; the systems of b (its mask) take part.
+std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ysc, int64_t rs,
+                                          int64_t ryd, bool x0, int64_t dx, int64_t dsv,
+                                          int64_t dyc, int64_t dyd, const dev::Batch& b,
+                                          hipEvent_t after_cg)
 {
   Sym& S = *sym_;
+  const Slots& z = S.z;
   const int n = S.n, m = S.m, d = S.d;
   // r^x = r~x + gamma Jc^T ry in the factor's internal order
-  dev::gather(S.tmp_n.p, rxt, S.phat_d.p, n, st_);
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, gamma_, ysc, 1.0, S.tmp_n.p, w.rxp,
-            st_);
-  dev::copy(S.t1.p, w.rxp, n, st_);
-  chol_solve(S.t1.p);
-  dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, S.jch_v.p, 1.0, S.t1.p, -1.0, ysc, w.schur, st_);
+  dev::gather(P(z.tmp_n), P(rxt), S.phat_d.p, n, b, st_);
+  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), 0.0, P(z.gam), P(ysc), 1.0,
+            P(z.tmp_n), P(w.rxp), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  dev::copy(P(z.t1), P(w.rxp), n, b, st_);
+  S.chol->solve(P(z.t1), b);
+  dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, P(z.jch_v), 1.0, nullptr, P(z.t1), -1.0, P(ysc),
+            P(w.schur), S.jch_blk.p, S.jch_nblk, b, st_);
 
   SchurOps ops;
   ops.m = m;
   ops.n = n;
   ops.jh_rp = S.jch_rp.p;
   ops.jh_ci = S.jch_ci.p;
-  ops.jh_v = S.jch_v.p;
+  ops.jh_blk = S.jch_blk.p;
+  ops.jh_nblk = S.jch_nblk;
+  ops.jh_v = P(z.jch_v);
   ops.jth_rp = S.jcth_rp.p;
   ops.jth_ci = S.jcth_ci.p;
-  ops.jth_v = S.jcth_v.p;
+  ops.jth_blk = S.jcth_blk.p;
+  ops.jth_nblk = S.jcth_nblk;
+  ops.jth_v = P(z.jcth_v);
   ops.chol = S.chol.get();
   ops.st = st_;
-  ops.t1 = S.t1.p;
-  ops.x = w.cx;
-  ops.r = S.cr.p;
-  ops.w = S.cw.p;
-  ops.p = S.cp.p;
-  ops.s = S.cs.p;
-  ops.sc = S.sc.p;
-  ops.part = S.part.p;
-  if (x0) dev::copy(w.cx, x0, m, st_);
-  else dev::fill(w.cx, 0.0, m, st_);
-  *cg_status = schur_cg_run(ops, w.schur, opts.cg_tol, opts.cg_itmax, opts.cg_check_every, iters,
-                            fnorm, x0 == nullptr);
+  ops.t1 = P(z.t1);
+  ops.x = P(w.cx);
+  ops.r = P(z.cr);
+  ops.w = P(z.cw);
+  ops.p = P(z.cp);
+  ops.s = P(z.cs);
+  ops.sc = P(z.sc);
+  ops.part = P(z.part);
+  ops.live = S.flag(FL_LIVE);
+  ops.iters = S.flag(FL_CG_IT);
+  ops.status = S.flag(FL_CG_ST);
+  ops.gath = S.gath.p;
+  if (x0) dev::copy(P(w.cx), P(z.in[IN_Y0]), m, b, st_);
+  else dev::fill(P(w.cx), 0.0, m, b, st_);
+  // the host looks at the live flags first after as many iterations as the
+  // last pass took (or every cg_check_every iterations when that is set)
+  const int chunk = opts.cg_check_every > 0 ? opts.cg_check_every : std::max(1, last_iters_);
+  const int chunk_next = opts.cg_check_every > 0 ? opts.cg_check_every : 1;
+  std::vector<CgResult> res =
+      schur_cg_run(ops, P(w.schur), opts.cg_tol, opts.cg_itmax, b, !x0, chunk, chunk_next);
   if (after_cg) HIPCHK(hipEventRecord(after_cg, st_));
 
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, S.jcth_v.p, -1.0, w.cx, 1.0, w.rxp, S.t1.p,
-            st_);
-  chol_solve(S.t1.p);
-  dev::gather_scale(dx, S.t1.p, S.ipos_d.p, S.max_d.p, n, st_);
-  dev::copy(dyc, w.cx, m, st_);
-  dev::vec_scale(dyc, S.max_d.p + n, m, st_);
-  dev::recover_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, dx, ryd, S.dsd, rs, dsv, dyd,
-                 st_);
+  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), -1.0, nullptr, P(w.cx), 1.0,
+            P(w.rxp), P(z.t1), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  S.chol->solve(P(z.t1), b);
+  dev::gather_scale(P(dx), P(z.t1), S.ipos_d.p, P(z.max_d), n, b, st_);
+  dev::copy(P(dyc), P(w.cx), m, b, st_);
+  dev::vec_scale(P(dyc), P(z.max_d) + n, m, b, st_);
+  dev::recover_d(d, S.jd_rp.p, S.jd_ci.p, P(z.in[IN_JD]), S.has_jd ? 1 : 0, P(dx), P(ryd),
+                 P(z.in[IN_DS]), P(rs), P(dsv), P(dyd), b, st_);
+  return res;
 }
 
 // KKT residual of (dx, dsv, dyc, dyd) against the bound right-hand sides:
-// out[0..3] = squared norms of rows 1, 3, 2, 4; out[4..7] = squared norms of
+// res[0..3] = squared norms of rows 1, 3, 2, 4; res[4..7] = squared norms of
 // rx, rs, ry, ryd.  rows: also keep the signed rows in e_x, e_y, e_s, e_d.
-// Synchronizes the stream.
-void Solver::residual(const double* dx, const double* dsv, const double* dyc, const double* dyd,
-                      bool rows, hipEvent_t done, double out[8])
+void Solver::residual(int64_t dx, int64_t dsv, int64_t dyc, int64_t dyd, bool rows,
+                      const dev::Batch& b)
 {
   Sym& S = *sym_;
+  const Slots& z = S.z;
   const int n = S.n, m = S.m, d = S.d;
-  dev::resid_rows_n(n, S.h_rp.p, S.h_ci.p, S.hv, S.jct_rp.p, S.jct_ci.p, S.jct_map.p, S.jcv,
-                    S.has_jd ? S.jdt_rp.p : nullptr, S.jdt_ci.p, S.jdt_map.p, S.jdv, S.drx, dx,
-                    dyc, dyd, S.sq_n.p, rows ? S.e_x.p : nullptr, st_);
-  dev::resid_rows_m(m, S.jc_rp.p, S.jc_ci.p, S.jcv, S.dry, dx, S.sq_m.p,
-                    rows ? S.e_y.p : nullptr, st_);
-  dev::resid_rows_d(d, S.jd_rp.p, S.jd_ci.p, S.jdv, S.has_jd ? 1 : 0, S.dsd, S.drs, S.dryd, dx,
-                    dsv, dyd, S.sq_d2.p, S.sq_d4.p, rows ? S.e_s.p : nullptr,
-                    rows ? S.e_d.p : nullptr, st_);
-  double* r = S.res.p;
-  dev::dot(n, S.sq_n.p, nullptr, S.part.p, r + 0, st_);
-  dev::dot(m, S.sq_m.p, nullptr, S.part.p, r + 1, st_);
-  dev::dot(d, S.sq_d2.p, nullptr, S.part.p, r + 2, st_);
-  dev::dot(d, S.sq_d4.p, nullptr, S.part.p, r + 3, st_);
-  dev::dot(n, S.drx, S.drx, S.part.p, r + 4, st_);
-  dev::dot(d, S.drs, S.drs, S.part.p, r + 5, st_);
-  dev::dot(m, S.dry, S.dry, S.part.p, r + 6, st_);
-  dev::dot(d, S.dryd, S.dryd, S.part.p, r + 7, st_);
-  if (done) HIPCHK(hipEventRecord(done, st_));
-  HIPCHK(hipMemcpyAsync(out, r, 8 * sizeof(double), hipMemcpyDeviceToHost, st_));
-  HIPCHK(hipStreamSynchronize(st_));
+  const double* in[IN_COUNT];
+  for (int j = 0; j < IN_COUNT; j++) in[j] = P(z.in[j]);
+  dev::resid_rows_n(n, S.h_rp.p, S.h_ci.p, in[IN_H], S.jct_rp.p, S.jct_ci.p, S.jct_map.p,
+                    in[IN_JC], S.has_jd ? S.jdt_rp.p : nullptr, S.jdt_ci.p, S.jdt_map.p, in[IN_JD],
+                    in[IN_RX], P(dx), P(dyc), P(dyd), P(z.sq_n), rows ? P(z.e_x) : nullptr, b, st_);
+  dev::resid_rows_m(m, S.jc_rp.p, S.jc_ci.p, in[IN_JC], in[IN_RY], P(dx), P(z.sq_m),
+                    rows ? P(z.e_y) : nullptr, b, st_);
+  dev::resid_rows_d(d, S.jd_rp.p, S.jd_ci.p, in[IN_JD], S.has_jd ? 1 : 0, in[IN_DS], in[IN_RS],
+                    in[IN_RYD], P(dx), P(dsv), P(dyd), P(z.sq_d2), P(z.sq_d4),
+                    rows ? P(z.e_s) : nullptr, rows ? P(z.e_d) : nullptr, b, st_);
+  double* r = P(z.res);
+  double* part = P(z.part);
+  dev::dot(n, P(z.sq_n), nullptr, part, r + 0, b, st_);
+  dev::dot(m, P(z.sq_m), nullptr, part, r + 1, b, st_);
+  dev::dot(d, P(z.sq_d2), nullptr, part, r + 2, b, st_);
+  dev::dot(d, P(z.sq_d4), nullptr, part, r + 3, b, st_);
+  dev::dot(n, in[IN_RX], in[IN_RX], part, r + 4, b, st_);
+  dev::dot(d, in[IN_RS], in[IN_RS], part, r + 5, b, st_);
+  dev::dot(m, in[IN_RY], in[IN_RY], part, r + 6, b, st_);
+  dev::dot(d, in[IN_RYD], in[IN_RYD], part, r + 7, b, st_);
 }
 
-void Solver::get_solution(double* dx, double* ds, double* dyc, double* dyd)
+std::vector<double> Solver::read_res(int count)
 {
-  HYKKT_REQUIRE(sym_ && stats.n_solves > 0, HYKKT_E_STATE, "no solution yet");
   Sym& S = *sym_;
-  auto d2h = [&](double* dst, const DevBuf<double>& b, int cnt) {
-    if (dst && cnt) HIPCHK(hipMemcpy(dst, b.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
-  };
-  d2h(dx, S.dx, S.n);
-  d2h(ds, S.dsv, S.d);
-  d2h(dyc, S.dyc, S.m);
-  d2h(dyd, S.dyd, S.d);
+  std::vector<double> h(static_cast<size_t>(RES_SLOTS) * count);
+  dev::collect(S.gath.p, P(S.z.res), RES_SLOTS, dev::Batch{count, S.z.stride, nullptr}, st_);
+  HIPCHK(hipMemcpyAsync(h.data(), S.gath.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                        st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return h;
 }
 
-void Solver::get_solution_device(const double** dx, const double** ds, const double** dyc,
-                                 const double** dyd) const
+void Solver::get_solution(int k, double* dx, double* ds, double* dyc, double* dyd)
 {
-  HYKKT_REQUIRE(sym_ && stats.n_solves > 0, HYKKT_E_STATE, "no solution yet");
-  if (dx) *dx = sym_->dx.p;
-  if (ds) *ds = sym_->dsv.p;
-  if (dyc) *dyc = sym_->dyc.p;
-  if (dyd) *dyd = sym_->dyd.p;
+  HYKKT_REQUIRE(sym_ && st_k_[0].n_solves > 0, HYKKT_E_STATE, "no solution yet");
+  HYKKT_REQUIRE(k >= 0 && k < sym_->cap, HYKKT_E_ARG, "system index beyond the batch capacity");
+  Sym& S = *sym_;
+  const Slots& z = S.z;
+  const int64_t o = static_cast<int64_t>(k) * z.stride;
+  auto d2h = [&](double* dst, int64_t off, int cnt) {
+    if (dst && cnt)
+      HIPCHK(hipMemcpy(dst, P(o + off), cnt * sizeof(double), hipMemcpyDeviceToHost));
+  };
+  d2h(dx, z.dx, S.n);
+  d2h(ds, z.dsv, S.d);
+  d2h(dyc, z.dyc, S.m);
+  d2h(dyd, z.dyd, S.d);
+}
+
+void Solver::get_solution_device(int k, const double** dx, const double** ds,
+                                 const double** dyc, const double** dyd) const
+{
+  HYKKT_REQUIRE(sym_ && st_k_[0].n_solves > 0, HYKKT_E_STATE, "no solution yet");
+  HYKKT_REQUIRE(k >= 0 && k < sym_->cap, HYKKT_E_ARG, "system index beyond the batch capacity");
+  const Slots& z = sym_->z;
+  const int64_t o = static_cast<int64_t>(k) * z.stride;
+  if (dx) *dx = P(o + z.dx);
+  if (ds) *ds = P(o + z.dsv);
+  if (dyc) *dyc = P(o + z.dyc);
+  if (dyd) *dyd = P(o + z.dyd);
 }
 
 void Solver::get_perm(int* perm, int n) const
@@ -677,6 +911,8 @@ long long Solver::debug_copy(int what, void* out, long long cap)
 {
   HYKKT_REQUIRE(sym_, HYKKT_E_STATE, "no analysis yet");
   Sym& S = *sym_;
+  const Slots& z = S.z;
+  const dev::Batch one{1, z.stride, nullptr};
   HIPCHK(hipStreamSynchronize(st_));
   auto dcopy = [&](const double* src, long long cnt) -> long long {
     HYKKT_REQUIRE(cnt <= cap, HYKKT_E_ARG, "debug_copy: buffer too small");
@@ -691,13 +927,13 @@ long long Solver::debug_copy(int what, void* out, long long cap)
   };
   DevBuf<double> tmp;
   switch (what) {
-    case 1: return dcopy(S.htil.p, S.htp.e.nnz());
+    case 1: return dcopy(P(z.htil), S.htp.e.nnz());
     case 2: return icopy(S.htp.e.rp);
     case 3: return icopy(S.htp.e.ci);
-    case 4: return dcopy(S.max_d.p, S.n + S.m);
-    case 5: return dcopy(S.jcs.p, S.nnz_jc);
-    case 6: return dcopy(S.rx_til.p, S.n);
-    case 7: return dcopy(S.ry_s.p, S.m);
+    case 4: return dcopy(P(z.max_d), S.n + S.m);
+    case 5: return dcopy(P(z.jcs), S.nnz_jc);
+    case 6: return dcopy(P(z.rx_til), S.n);
+    case 7: return dcopy(P(z.ry_s), S.m);
     case 8: return icopy(S.hgp.e.rp);
     case 9: return icopy(S.hgp.e.ci);
     case 10: {
@@ -710,7 +946,7 @@ long long Solver::debug_copy(int what, void* out, long long cap)
       CholEngine& ch = *S.chol;
       const int na = ch.n_assembled();
       tmp.alloc(std::max(1, na));
-      assemble_hgamma();
+      assemble_hgamma(one);
       dev::gather64(tmp.p, ch.L(), ch.a_dst(), na, st_);
       std::vector<double> got(std::max(1, na));
       HIPCHK(hipMemcpyAsync(got.data(), tmp.p, sizeof(double) * na, hipMemcpyDeviceToHost, st_));
@@ -724,22 +960,22 @@ long long Solver::debug_copy(int what, void* out, long long cap)
     case 11:   // H~ before Ruiz, recomputed from the current inputs
       tmp.alloc(std::max(1, S.htp.e.nnz()));
       if (S.has_jd)
-        dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, S.hv, S.ht_tp.p, S.ht_ta.p, S.ht_tb.p,
-                           S.jdv, S.jd_vs.p, 1.0, tmp.p, st_);
+        dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, P(z.in[IN_H]), S.ht_tp.p, S.ht_ta.p,
+                           S.ht_tb.p, P(z.in[IN_JD]), P(z.jd_vs), 1.0, tmp.p, one, st_);
       else
-        dev::copy(tmp.p, S.hv, S.nnz_h, st_);
+        dev::copy(tmp.p, P(z.in[IN_H]), S.nnz_h, one, st_);
       HIPCHK(hipStreamSynchronize(st_));
       return dcopy(tmp.p, S.htp.e.nnz());
     case 12: {  // r^x in natural order
       std::vector<double> a(S.n), b(S.n);
-      HIPCHK(hipMemcpy(a.data(), S.rxp.p, S.n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(a.data(), P(z.rxp), S.n * sizeof(double), hipMemcpyDeviceToHost));
       for (int k = 0; k < S.n; k++) b[S.phat[k]] = a[k];
       HYKKT_REQUIRE(S.n <= cap, HYKKT_E_ARG, "debug_copy: buffer too small");
       std::memcpy(out, b.data(), S.n * sizeof(double));
       return S.n;
     }
-    case 13: return dcopy(S.schur.p, S.m);
-    case 14: return dcopy(S.cx.p, S.m);
+    case 13: return dcopy(P(z.schur), S.m);
+    case 14: return dcopy(P(z.cx), S.m);
     case 15: return icopy(S.hgp_pat.rp);
     case 16: return icopy(S.hgp_pat.ci);
     case 17: return icopy(S.hgp_map);

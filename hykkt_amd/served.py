@@ -1,26 +1,28 @@
 """Served form: continuous batching of KKT solve requests over resident
 sparsity patterns, beside the plain one-request-at-a-time path.
 
-Device state.  Every pattern keeps its solver slots resident: a slot is one
-HykktSolver (symbolic analysis, permutation, factor storage and workspace, and
-its own HIP streams).  Slots are created up front, largest patterns first,
-while the device memory they take (measured with hipMemGetInfo around each
-slot's first solve) fits the HBM budget; a pattern that gets no slot is not
-served.  The request values (perturbed H, Jc, Jd values, Ds, right-hand sides)
-come from a pool per pattern that is resident in HBM, so the request path has
-no host-to-device copies (inputs_on_device).
+Device state.  Every admitted pattern keeps a batch slot resident: a
+BatchSolver (symbolic analysis, permutation, and per-system storage for up to
+`capacity` systems: factor panels, update matrices, workspace) with its own
+HIP streams.  Slots are created up front, largest patterns first, while the
+device memory they take (measured with hipMemGetInfo around each slot's first
+batch) fits the HBM budget; a pattern that gets no slot is not served and its
+requests are rejected.  Request values (perturbed H, Jc, Jd values, Ds,
+right-hand sides) come from a pool per pattern resident in HBM, so the
+request path copies nothing from the host.
 
-Serving.  Each slot has a host worker thread.  Requests wait in their
-pattern's FIFO (bounded: arrivals beyond it are rejected and counted) and the
-first free slot of the pattern takes the next one.  A request arriving while
-others are on the GPU is admitted as soon as a slot of its pattern frees up,
-and the work of different slots overlaps on the device: each slot's kernels
-run on its own streams.  A solve's arithmetic does not depend on what else is
-running (every slot has its own buffers and the kernels are deterministic),
-so a result is the same alone and in the mix.
+Serving.  Requests wait in their pattern's FIFO (bounded: arrivals beyond it
+are rejected and counted).  Each slot has a host worker: whenever the slot is
+free it takes every request of its pattern that has arrived, up to the slot's
+capacity, and solves them as one batch -- every kernel launch covers all of
+them.  Requests that arrive while a batch is on the GPU join the next one, so
+batches grow with the load (continuous batching), and the slots of different
+patterns overlap on the device.  A system's arithmetic in a batch is exactly
+its arithmetic alone, so a result is the same bits alone and in the mix.
 
 Plain.  One worker takes the requests one at a time in arrival order over all
-patterns, on the same resident solvers: the drop-in path run back to back.
+patterns, each through the drop-in HykktSolver.execute() of its pattern's
+plain solver.
 
 Non-positive pivots.  A system whose H_gamma is not positive definite (H~ not
 positive definite on the null space of Jc) stops the factorization at a
@@ -32,7 +34,7 @@ from __future__ import annotations
 import queue
 import threading
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -41,6 +43,19 @@ from hykkt_amd.generator import GridPattern, ds_spread_schedule
 
 # Bus counts of the served mix and their shares (6:3:1).
 SERVED_SIZES = ((1_000, 0.6), (5_000, 0.3), (20_000, 0.1))
+# Value sets per served pattern: request i of a pattern uses entry i mod POOL,
+# whose Ds spread grows with the entry (the interior-point step index).
+POOL = 20
+
+
+def default_capacity(nbus: int) -> int:
+    """Systems per batch slot: small patterns are latency-bound, so many of
+    them share each launch; large ones fill the device on their own."""
+    if nbus <= 2_000:
+        return 16
+    if nbus <= 8_000:
+        return 8
+    return 2
 
 
 @dataclass
@@ -61,6 +76,8 @@ class Result:
     solution: tuple | None = None
     k: int = -1          # the pattern's value set
     pivot: int = -1      # first non-positive pivot of the factorization (-1: none)
+    batch: int = 1       # systems in the batch that solved it
+    t_arrive: float = 0.0
 
 
 class ServedPattern:
@@ -69,7 +86,7 @@ class ServedPattern:
     spread growing with the pool index up to 1e-4..1e8, or `ds_ranges[k]`;
     gamma `gamma`, or `gammas[k]`, for entry k)."""
 
-    def __init__(self, pid: int, nbus: int, seed: int, pool: int = 2, gamma: float = 1e5,
+    def __init__(self, pid: int, nbus: int, seed: int, pool: int = POOL, gamma: float = 1e5,
                  ds_ranges=None, gammas=None):
         self.pid, self.nbus = pid, nbus
         self.gammas = [float(gamma)] * pool if gammas is None else [float(x) for x in gammas]
@@ -90,7 +107,8 @@ class ServedPattern:
                                   rx=f(sy.rx), rs=f(sy.rs), ry=f(sy.ry), ryd=f(sy.ryd)))
             self.host.append(sy)
 
-    def bind(self, s: "H.HykktSolver", k: int):
+    def set_pattern(self, s: "H.HykktSolver", k: int = 0):
+        """The pattern (and, for the drop-in path, the values of entry k)."""
         g, v, ix = self.g, self.pool[k], self.idx
         s.set_gamma(self.gammas[k])
         s.set_H(ix["Hrp"], ix["Hci"], v["H"], g.n, g.n, g.H.nnz)
@@ -102,6 +120,14 @@ class ServedPattern:
         s.set_ry(v["ry"], g.m)
         s.set_ryd(v["ryd"], g.d)
 
+    bind = set_pattern
+
+    def bind_system(self, b: "H.BatchSolver", slot: int, k: int):
+        """Pool entry k as system `slot` of batch solver b."""
+        v = self.pool[k]
+        b.bind_system(slot, self.gammas[k], v["H"], v["Ds"], v["Jc"], v["Jd"], v["rx"], v["rs"],
+                      v["ry"], v["ryd"])
+
     def free(self):
         for a in self.idx.values():
             a.free()
@@ -110,7 +136,7 @@ class ServedPattern:
                 a.free()
 
 
-def make_patterns(npat: int = 16, seed: int = 0, pool: int = 2, sizes=SERVED_SIZES):
+def make_patterns(npat: int = 16, seed: int = 0, pool: int = POOL, sizes=SERVED_SIZES):
     """npat patterns with bus counts in the served mix's shares (largest
     remainder rounding), seeds seed, seed+1, ..."""
     counts = [int(np.floor(npat * p)) for _, p in sizes]
@@ -126,57 +152,67 @@ def make_patterns(npat: int = 16, seed: int = 0, pool: int = 2, sizes=SERVED_SIZ
 
 
 class Server:
-    """Solver slots for `patterns` within `hbm_budget` bytes; see the module
-    docstring.  slots_per_pattern: slots created per pattern when the budget
-    allows (at least one each while it does)."""
+    """A batch slot and a plain solver for each of `patterns` within
+    `hbm_budget` bytes; see the module docstring.  capacity: systems per batch
+    slot (a function of the bus count, default_capacity by default)."""
 
-    def __init__(self, patterns, hbm_budget: float, slots_per_pattern: int = 1,
-                 queue_limit: int = 4096, keep_solutions: bool = False):
+    def __init__(self, patterns, hbm_budget: float, capacity=None, queue_limit: int = 4096,
+                 keep_solutions: bool = False):
         self.patterns = {p.pid: p for p in patterns}
         self.keep = keep_solutions
         self.queue_limit = queue_limit
-        self.slots = {p.pid: [] for p in patterns}
+        self.slots = {}      # pid -> BatchSolver
+        self.plain = {}      # pid -> HykktSolver (the drop-in path)
         self.failures = []   # messages of solves the library reported as failed
-        self.refused = []    # slots not kept: failed first solve or over the budget
+        self.refused = []    # patterns not kept: failed first solve or over the budget
         self.hbm_used = 0.0
-        order = sorted(patterns, key=lambda p: -p.nbus)
-        for _ in range(slots_per_pattern):
-            for p in order:
-                free0, _ = H.device_mem_info()
-                s = H.HykktSolver(p.g.gamma)
-                s.options(inputs_on_device=1)
-                # symbolic analysis + first solve: the slot's full footprint; a
-                # value set with a non-positive pivot still sizes the slot, and
-                # a slot the device cannot hold is refused
-                p.bind(s, 0)
-                try:
-                    rc = s.execute()
-                except H.HykktError as e:
-                    self.refused.append({"pattern": p.pid, "nbus": p.nbus, "error": str(e)})
-                    s.close()
-                    continue
-                if rc == H.HYKKT_RESIDUAL_FAIL and s.stats()["pivot"] >= 0:
-                    rc = 0
-                H.device_synchronize()
-                free1, _ = H.device_mem_info()
-                used = max(free0 - free1, 0)
-                if rc != 0 or self.hbm_used + used > hbm_budget:
-                    st = s.stats()
-                    self.refused.append({"pattern": p.pid, "nbus": p.nbus, "rc": rc,
-                                         "bytes": used, "resid2": st["resid2"],
-                                         "resid4": st["resid4"], "pivot": st["pivot"]})
-                    s.close()
-                    continue
-                self.hbm_used += used
-                self.slots[p.pid].append(s)
-        self.served = [p.pid for p in patterns if self.slots[p.pid]]
-        # each pattern's solve time alone (second solve on its first slot,
-        # pool entry 1): the unit for weighting completions by their work
-        # (a solve that fails there keeps the pattern out of the weighting)
+        cap_of = capacity if callable(capacity) else (
+            (lambda nbus: capacity) if capacity else default_capacity)
+        self.capacity = {}
+        for p in sorted(patterns, key=lambda p: -p.nbus):
+            cap = max(1, min(cap_of(p.nbus), len(p.pool)))
+            free0, _ = H.device_mem_info()
+            b = H.BatchSolver(p.g.gamma, cap)
+            b.options(inputs_on_device=1)
+            s = H.HykktSolver(p.g.gamma)
+            s.options(inputs_on_device=1)
+            p.set_pattern(b, 0)
+            p.set_pattern(s, 0)
+            # symbolic analysis + a first full batch and a first plain solve:
+            # the slot's footprint; a value set with a non-positive pivot still
+            # sizes the slot, and a slot the device cannot hold is refused
+            try:
+                for j in range(cap):
+                    p.bind_system(b, j, j % len(p.pool))
+                rcs = b.execute_batch(cap)
+                rc = s.execute()
+            except H.HykktError as e:
+                self.refused.append({"pattern": p.pid, "nbus": p.nbus, "error": str(e)})
+                b.close()
+                s.close()
+                continue
+            H.device_synchronize()
+            free1, _ = H.device_mem_info()
+            used = max(free0 - free1, 0)
+            bad = [r for j, r in enumerate(rcs) if r < 0]
+            if bad or rc < 0 or self.hbm_used + used > hbm_budget:
+                self.refused.append({"pattern": p.pid, "nbus": p.nbus, "rc": [rc] + rcs,
+                                     "bytes": used})
+                b.close()
+                s.close()
+                continue
+            self.hbm_used += used
+            self.slots[p.pid] = b
+            self.plain[p.pid] = s
+            self.capacity[p.pid] = cap
+        self.served = [p.pid for p in patterns if p.pid in self.slots]
+        # each pattern's solve time alone (a second plain solve, pool entry 1):
+        # the unit for weighting completions by their work (a solve that fails
+        # there keeps the pattern out of the weighting)
         self.solo_s = {}
         for pid in self.served:
-            s = self.slots[pid][0]
-            self.patterns[pid].bind(s, 1 % len(self.patterns[pid].pool))
+            s = self.plain[pid]
+            self.patterns[pid].set_pattern(s, 1 % len(self.patterns[pid].pool))
             t = time.perf_counter()
             try:
                 rc = s.execute()
@@ -188,26 +224,50 @@ class Server:
                 self.solo_s[pid] = dt
 
     def close(self):
-        for ss in self.slots.values():
-            for s in ss:
+        for ss in (self.slots, self.plain):
+            for s in ss.values():
                 s.close()
-        self.slots = {}
+        self.slots, self.plain = {}, {}
 
-    # ---- one run over a request list ----
-    def _solve(self, s, req: Request) -> Result:
-        """One request on slot `s`.  A solve the library reports as failed
-        (an error code) is a result with that code and an infinite residual;
-        the slot goes on with the next request."""
+    # ---- solving ----
+    def _solve_plain(self, req: Request) -> Result:
+        """One request through the drop-in path of its pattern.  A solve the
+        library reports as failed (an error code) is a result with that code
+        and an infinite residual; the worker goes on with the next request."""
         p = self.patterns[req.pattern]
+        s = self.plain[req.pattern]
         try:
-            p.bind(s, req.k)
+            p.set_pattern(s, req.k)
             rc = s.execute()
         except H.HykktError as e:
             self.failures.append(str(e))
-            return Result(req.rid, req.pattern, 0.0, e.code, float("inf"), None)
+            return Result(req.rid, req.pattern, 0.0, e.code, float("inf"), None, req.k)
         st = s.stats()
         sol = tuple(np.copy(a) for a in s.solution()) if self.keep else None
         return Result(req.rid, req.pattern, 0.0, rc, st["resid4"], sol, req.k, st["pivot"])
+
+    def _solve_batch(self, reqs) -> list:
+        """Requests of one pattern as one batch on its slot."""
+        pid = reqs[0].pattern
+        p, b = self.patterns[pid], self.slots[pid]
+        try:
+            for j, r in enumerate(reqs):
+                p.bind_system(b, j, r.k)
+            rcs = b.execute_batch(len(reqs))
+        except H.HykktError as e:
+            self.failures.append(str(e))
+            return [Result(r.rid, pid, 0.0, e.code, float("inf"), None, r.k, -1, len(reqs))
+                    for r in reqs]
+        out = []
+        for j, r in enumerate(reqs):
+            st = b.stats_k(j)
+            sol = tuple(np.copy(a) for a in b.solution_k(j)) if self.keep else None
+            rc = rcs[j]
+            if rc < 0:
+                self.failures.append(f"request {r.rid}: status {rc}")
+            out.append(Result(r.rid, pid, 0.0, rc, st["resid4"] if rc >= 0 else float("inf"),
+                              sol, r.k, st["pivot"], len(reqs)))
+        return out
 
     def run(self, requests, plain: bool = False, deadline: float | None = None):
         """Serve `requests` (arrival times relative to the start, sorted).
@@ -221,12 +281,15 @@ class Server:
         rejected = [0]
         if plain:
             queues = {None: queue.Queue()}
-            workers = [(None, [self.slots[pid][0] for pid in self.served])]
+            keys = [None]
         else:
             queues = {pid: queue.Queue() for pid in self.served}
-            workers = [(pid, [s]) for pid in self.served for s in self.slots[pid]]
+            keys = list(self.served)
 
-        def worker(key, solvers):
+        def late():
+            return deadline is not None and time.perf_counter() - t0 > deadline
+
+        def worker(key):
             q = queues[key]
             while True:
                 try:
@@ -237,16 +300,33 @@ class Server:
                     continue
                 if req is None:
                     return
-                if deadline is not None and time.perf_counter() - t0 > deadline:
+                if late():
                     continue   # past the deadline: pending, not run
-                s = solvers[0] if key is not None else \
-                    self.slots[req.pattern][0]
-                r = self._solve(s, req)
-                r.latency_s = time.perf_counter() - t0 - req.t_arrive
+                if key is None:
+                    res = [self._solve_plain(req)]
+                else:
+                    batch, done = [req], False
+                    while len(batch) < self.capacity[key]:
+                        try:
+                            nxt = q.get_nowait()
+                        except queue.Empty:
+                            break
+                        if nxt is None:
+                            done = True
+                            break
+                        batch.append(nxt)
+                    res = self._solve_batch(batch)
+                t = time.perf_counter() - t0
+                reqs = {r.rid: r for r in ([req] if key is None else batch)}
+                for r in res:
+                    r.t_arrive = reqs[r.rid].t_arrive
+                    r.latency_s = t - r.t_arrive
                 with lock:
-                    results.append(r)
+                    results.extend(res)
+                if key is not None and done:
+                    return
 
-        threads = [threading.Thread(target=worker, args=w, daemon=True) for w in workers]
+        threads = [threading.Thread(target=worker, args=(k,), daemon=True) for k in keys]
         for t in threads:
             t.start()
         arrived = 0
@@ -254,7 +334,7 @@ class Server:
             dt = req.t_arrive - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            if deadline is not None and time.perf_counter() - t0 > deadline:
+            if late():
                 break
             arrived += 1
             key = None if plain else req.pattern
@@ -263,11 +343,10 @@ class Server:
                 continue
             queues[key].put(req)
         if deadline is not None:
-            while time.perf_counter() - t0 < deadline and any(q.qsize() for q in queues.values()):
+            while not late() and any(q.qsize() for q in queues.values()):
                 time.sleep(0.01)
         for q in queues.values():
-            for _ in range(len(threads)):
-                q.put(None)
+            q.put(None)
         stop.set()
         for t in threads:
             t.join()
@@ -276,33 +355,41 @@ class Server:
         return results, rejected[0], pending, wall
 
 
-def poisson_requests(patterns, rate: float, seconds: float, seed: int = 0, pool: int = 2,
+def poisson_requests(patterns, rate: float, seconds: float, seed: int = 0, pool: int = POOL,
                      served=None):
     """Poisson arrivals at `rate` per second for `seconds`; each picks a
-    pattern in the served mix's shares and a pool entry."""
+    pattern in the served mix's shares.  A pattern's i-th request is its
+    interior-point step i: pool entry i mod pool, whose Ds spread grows with
+    the entry."""
     rng = np.random.default_rng(seed)
     ids = served if served is not None else [p.pid for p in patterns]
-    sizes = np.array([patterns[i].nbus for i in ids])
+    by_id = {p.pid: p for p in patterns}
+    sizes = np.array([by_id[i].nbus for i in ids])
     share = {n: p for n, p in SERVED_SIZES}
     w = np.array([share.get(int(n), 1.0) / max(np.sum(sizes == n), 1) for n in sizes])
     w /= w.sum()
     t, reqs, rid = 0.0, [], 0
+    step = {i: 0 for i in ids}
     while True:
         t += rng.exponential(1.0 / rate)
         if t >= seconds:
             break
-        reqs.append(Request(rid, int(ids[rng.choice(len(ids), p=w)]), int(rng.integers(pool)), t))
+        pid = int(ids[rng.choice(len(ids), p=w)])
+        reqs.append(Request(rid, pid, step[pid] % pool, t))
+        step[pid] += 1
         rid += 1
     return reqs
 
 
-def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=None):
+def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=None,
+              warmup=0.0):
     """Counts, throughput (results within `tol` per wall second) and latency
-    quantiles; with `patterns`, also per bus count.  With `solo_s` (seconds
-    of each pattern's solve run alone), also the solo seconds of the results
-    within `tol` per wall second: a throughput that does not depend on which
-    sizes were completed (1.0 for one solve at a time, back to back)."""
-    lat = np.array([r.latency_s for r in results]) * 1e3
+    quantiles of the requests that arrived after `warmup` seconds; with
+    `patterns`, also per bus count.  With `solo_s` (seconds of each pattern's
+    solve run alone), also the solo seconds of the results within `tol` per
+    wall second: a throughput that does not depend on which sizes were
+    completed (1.0 for one solve at a time, back to back)."""
+    lat = np.array([r.latency_s for r in results if r.t_arrive >= warmup]) * 1e3
     ok = [r for r in results if r.rc == 0 and r.resid4 <= tol]
     out = {"completed": len(results), "within_tol": len(ok),
            "failed": sum(1 for r in results if r.rc != 0),
@@ -311,14 +398,17 @@ def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=
            "solves_per_s": len(ok) / wall if wall > 0 else 0.0,
            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
-           "max_resid4": max((r.resid4 for r in results), default=None)}
+           "max_resid4": max((r.resid4 for r in results), default=None),
+           "mean_batch": float(np.mean([r.batch for r in results])) if results else None,
+           "latency_after_s": warmup}
     if solo_s is not None:
         out["solo_s_per_s"] = (sum(solo_s[r.pattern] for r in ok if r.pattern in solo_s) / wall
                                if wall > 0 else 0.0)
     if patterns is not None:
         by = {}
-        for n in sorted({p.nbus for p in patterns}):
-            sel = [r for r in results if patterns[r.pattern].nbus == n]
+        pmap = patterns if isinstance(patterns, dict) else {p.pid: p for p in patterns}
+        for n in sorted({p.nbus for p in pmap.values()}):
+            sel = [r for r in results if pmap[r.pattern].nbus == n and r.t_arrive >= warmup]
             ls = np.array([r.latency_s for r in sel]) * 1e3
             by[str(n)] = {"completed": len(sel),
                           "p50_ms": float(np.percentile(ls, 50)) if len(ls) else None,

@@ -108,7 +108,10 @@ typedef struct {
   double delta;             /* diagonal shift added to H_gamma (add_diag hook,
                                HykktSolver.cpp:392); default 0 */
   int    inputs_on_device;  /* 1: pointers given to the setters are device pointers */
-  int    cg_check_every;    /* CG convergence read back every k iterations (default 1) */
+  int    cg_check_every;    /* the stopping test runs on the device every iteration; the host
+                               reads whether systems are still iterating every k iterations
+                               (k > 0), or first after as many iterations as the last solve
+                               took and then every iteration (0, the default) */
   int    verbose;           /* print the reference's progress lines (default 0) */
   int    refine_max;        /* iterative-refinement passes on the 4x4 KKT system after the
                                reference's single pass (default 3; 0 = the reference) */
@@ -149,6 +152,37 @@ int hykkt_get_solution(hykkt_solver s, double* dx, double* ds, double* dyc, doub
 /* Device pointers of the last solution (valid until the next execute/destroy). */
 int hykkt_get_solution_device(hykkt_solver s, const double** dx, const double** ds,
                               const double** dyc, const double** dyd);
+
+/* ---------------------------------------------------------------------------
+ * Batched form (no counterpart in the reference, whose execute() solves one
+ * system; its numeric refactorisation per system is HykktSolver.cpp:240-286,
+ * CholeskyClass.cpp:71-101).  A batch solver holds up to `capacity` systems
+ * that share one sparsity pattern: the pattern is given by the hykkt_set_H /
+ * _Ds / _Jc / _Jd setters (their value arrays are not used), each system's
+ * values by hykkt_bind_system, and hykkt_execute_batch solves systems
+ * 0 .. count-1 together, every kernel launch covering all of them.  Each
+ * system gets exactly the arithmetic it gets alone through hykkt_execute, so
+ * its results are bit-identical to a solve of that system alone.  Pointers
+ * follow the option inputs_on_device (device or host arrays) and are read at
+ * each hykkt_execute_batch.
+ * ------------------------------------------------------------------------- */
+int hykkt_create_batch(double gamma, int capacity, hykkt_solver* out);
+int hykkt_batch_capacity(hykkt_solver s, int* capacity);
+/* Values of system k < capacity: H, Ds, Jc, Jd values in the setters'
+ * patterns, the four right-hand sides, and the Schur CG start y0 (may be
+ * NULL: zero); jd_vals may be NULL when Jd is empty. */
+int hykkt_bind_system(hykkt_solver s, int k, double gamma, const double* h_vals,
+                      const double* ds_vals, const double* jc_vals, const double* jd_vals,
+                      const double* rx, const double* rs, const double* ry, const double* ryd,
+                      const double* y0);
+/* status[k] (count entries): what hykkt_execute would return for system k
+ * alone (0, 1, or a negative error code of that system, e.g. HYKKT_E_ARG for
+ * a row without a nonzero value). */
+int hykkt_execute_batch(hykkt_solver s, int count, int* status);
+int hykkt_get_stats_k(hykkt_solver s, int k, hykkt_stats* st);
+int hykkt_get_solution_k(hykkt_solver s, int k, double* dx, double* ds, double* dyc, double* dyd);
+int hykkt_get_solution_device_k(hykkt_solver s, int k, const double** dx, const double** ds,
+                                const double** dyc, const double** dyd);
 
 /* The fill-reducing ordering used (perm[k] = original index of pivot k). */
 int hykkt_get_permutation(hykkt_solver s, int* perm, int n);

@@ -67,25 +67,29 @@ def test_oracle_cg_itmax_reported():
 
 
 class _FakeSolver:
-    """Stands in for a solver slot in the admission logic: each slot's
-    first solve 'takes' the bytes its pattern asks for."""
+    """Stands in for a pattern's solvers in the admission logic: the batch
+    slot's first batch 'takes' the bytes its pattern asks for."""
     mem = {"free": 100e9}
 
-    def __init__(self, gamma):
-        self.need = 0
+    def __init__(self, gamma, capacity=1):
+        self.need, self.taken = 0, 0
 
     def options(self, **kw):
         return None
 
-    def execute(self):
+    def execute_batch(self, count):
+        self.taken = self.need
         _FakeSolver.mem["free"] -= self.need
+        return [0] * count
+
+    def execute(self):
         return 0
 
     def stats(self):
         return {"resid2": 1e-12, "resid4": 1e-12, "pivot": -1}
 
     def close(self):
-        _FakeSolver.mem["free"] += self.need
+        _FakeSolver.mem["free"] += self.taken
 
 
 class _FakePattern:
@@ -94,8 +98,11 @@ class _FakePattern:
         self.pool = [0, 1]
         self.g = type("G", (), {"gamma": 1e5})()
 
-    def bind(self, s, k):
-        s.need = self.need if k == 0 else 0
+    def set_pattern(self, s, k=0):
+        s.need = self.need
+
+    def bind_system(self, b, slot, k):
+        pass
 
 
 def test_server_refuses_a_slot_over_the_budget(monkeypatch):
@@ -105,6 +112,7 @@ def test_server_refuses_a_slot_over_the_budget(monkeypatch):
     from hykkt_amd import served as S
     _FakeSolver.mem["free"] = 100e9
     monkeypatch.setattr(S.H, "HykktSolver", _FakeSolver)
+    monkeypatch.setattr(S.H, "BatchSolver", _FakeSolver)
     monkeypatch.setattr(S.H, "device_mem_info", lambda: (_FakeSolver.mem["free"], 288e9))
     monkeypatch.setattr(S.H, "device_synchronize", lambda: None)
     pats = [_FakePattern(0, 1000, 2e9), _FakePattern(1, 20000, 30e9), _FakePattern(2, 5000, 9e9)]

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
-from hykkt_amd.batched import make_batch, run_batch, summarize_batch
+from hykkt_amd.batched import make_batch, make_server, run_batch, summarize_batch
 from hykkt_amd.served import Request, Server, ServedPattern, summarize
 
 pytestmark = pytest.mark.gpu
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def batch():
     pats = make_batch(npat=4, draws=3, seed=5, buses=(200, 800))
-    srv = Server(pats, hbm_budget=64e9, keep_solutions=True)
+    srv = make_server(pats, hbm_budget=64e9, keep_solutions=True)
     yield pats, srv
     srv.close()
     for p in pats:
@@ -47,7 +47,7 @@ def test_batch_result_matches_oracle(batch):
     assert sy.gamma == pats[2].gammas[1]
     c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
     ref = O.execute(c(sy.H), sy.Ds.v, c(sy.Jc), c(sy.Jd), sy.rx, sy.rs, sy.ry, sy.ryd, sy.gamma,
-                    perm=srv.slots[2][0].permutation())
+                    perm=srv.slots[2].permutation())
     mine = np.concatenate(r.solution)
     xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
     err = np.linalg.norm(mine - xo) / np.linalg.norm(xo)
@@ -79,13 +79,13 @@ def test_non_positive_pivot_reported_like_the_oracle():
         assert out["within_tol"] == 1 and out["failed"] == 1 and out["non_positive_pivot"] == 1
         sy = p.host[5]
         c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
-        perm = srv.slots[0][0].permutation()
+        perm = srv.slots[0].permutation()
         for g in (gs[5], 100.0 * gs[5]):
             ref = O.execute(c(sy.H), sy.Ds.v, c(sy.Jc), c(sy.Jd), sy.rx, sy.rs, sy.ry, sy.ryd,
                             float(g), perm=perm, chol_kind=1)
             assert ref["stats"]["pivot"] >= 0, g
-        s = srv.slots[0][0]
-        p.bind(s, 5)
+        s = srv.plain[0]
+        p.set_pattern(s, 5)
         s.set_gamma(100.0 * gs[5])
         assert s.execute() == 1 and s.stats()["pivot"] >= 0
     finally:

@@ -48,7 +48,7 @@ def test_served_result_matches_oracle(served):
     sy = p.host[k]
     c = lambda b: O.Csr(b.n, b.m, b.rp, b.ci, b.v)
     ref = O.execute(c(sy.H), sy.Ds.v, c(sy.Jc), c(sy.Jd), sy.rx, sy.rs, sy.ry, sy.ryd, sy.gamma,
-                    perm=srv.slots[0][0].permutation())
+                    perm=srv.slots[0].permutation())
     mine = np.concatenate(res[0].solution)
     xo = np.concatenate([ref["dx"], ref["ds"], ref["dyc"], ref["dyd"]])
     err = np.linalg.norm(mine - xo) / np.linalg.norm(xo)

@@ -17,7 +17,10 @@ def test_poisson_stream_mix_and_rate():
     sizes = np.array([pats[r.pattern].nbus for r in reqs])
     for n, share in ((1000, 0.6), (5000, 0.3), (20000, 0.1)):
         assert abs(np.mean(sizes == n) - share) < 0.02, n
-    assert set(r.k for r in reqs) == {0, 1}
+    # a pattern's i-th request is its interior-point step i: pool entry i mod 20
+    for pid in range(len(pats)):
+        ks = [r.k for r in reqs if r.pattern == pid]
+        assert ks == [i % 20 for i in range(len(ks))], pid
 
 
 def test_summary_counts():
@@ -29,8 +32,8 @@ def test_summary_counts():
 
 
 class _FakeSolver:
-    """Stands in for a solver slot: draw 1 of a pattern raises a library
-    error, draw 2 meets a non-positive pivot, others solve."""
+    """Stands in for a pattern's drop-in solver: draw 1 of a pattern raises a
+    library error, draw 2 meets a non-positive pivot, others solve."""
 
     def __init__(self):
         self.k = None
@@ -45,19 +48,39 @@ class _FakeSolver:
         return {"resid4": float("nan") if self.k == 2 else 1e-10, "pivot": 7 if self.k == 2 else -1}
 
 
+class _FakeBatch:
+    """Stands in for a batch slot: per system, draw 1 ends with an error code
+    of its own, draw 2 meets a non-positive pivot, others solve."""
+
+    def __init__(self):
+        self.ks = {}
+
+    def execute_batch(self, count):
+        return [(-5 if self.ks[j] == 1 else 1 if self.ks[j] == 2 else 0) for j in range(count)]
+
+    def stats_k(self, j):
+        k = self.ks[j]
+        return {"resid4": float("nan") if k == 2 else 1e-10, "pivot": 7 if k == 2 else -1}
+
+
 class _FakePattern:
     def __init__(self, pid, nbus):
         self.pid, self.nbus = pid, nbus
 
-    def bind(self, s, k):
+    def set_pattern(self, s, k=0):
         s.k = k
 
+    def bind_system(self, b, slot, k):
+        b.ks[slot] = k
 
-def _fake_server(npat=3):
+
+def _fake_server(npat=3, cap=4):
     from hykkt_amd.served import Server
     srv = object.__new__(Server)
     srv.patterns = {i: _FakePattern(i, 1000) for i in range(npat)}
-    srv.slots = {i: [_FakeSolver()] for i in range(npat)}
+    srv.plain = {i: _FakeSolver() for i in range(npat)}
+    srv.slots = {i: _FakeBatch() for i in range(npat)}
+    srv.capacity = {i: cap for i in range(npat)}
     srv.served = list(range(npat))
     srv.keep, srv.queue_limit, srv.failures, srv.refused = False, 4096, [], []
     return srv
@@ -66,7 +89,7 @@ def _fake_server(npat=3):
 def test_workers_survive_failed_requests():
     """A request the library rejects (an error code) or that meets a
     non-positive pivot is a failed result; the slot goes on with the next
-    request, in the served and the plain form alike."""
+    request, in the served (batched) and the plain form alike."""
     from hykkt_amd.served import Request
     reqs = [Request(i, i % 3, i % 4, 0.0) for i in range(24)]
     for plain in (False, True):
@@ -75,6 +98,11 @@ def test_workers_survive_failed_requests():
         assert rej == 0 and pend == 0 and len(res) == 24
         out = summarize(res, rej, pend, wall)
         assert out["failed"] == 12 and out["non_positive_pivot"] == 6 and out["within_tol"] == 12
-        assert len(srv.failures) == 6 and all("planted" in f for f in srv.failures)
+        assert len(srv.failures) == 6
+        if plain:
+            assert all("planted" in f for f in srv.failures)
+        else:
+            assert all("status -5" in f for f in srv.failures)
+            assert max(r.batch for r in res) > 1   # requests present together share a batch
         by = {r.rid: r for r in res}
         assert all(by[i].rc == (-5 if i % 4 == 1 else 1 if i % 4 == 2 else 0) for i in range(24))

@@ -41,7 +41,8 @@ class Options(C.Structure):
     _fields_ = [("ruiz_its", C.c_int), ("norm_tol", C.c_double), ("pivot_tol", C.c_double),
                 ("cg_tol", C.c_double), ("cg_itmax", C.c_int), ("delta", C.c_double),
                 ("inputs_on_device", C.c_int), ("cg_check_every", C.c_int),
-                ("verbose", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double)]
+                ("verbose", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double),
+                ("autotune", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -54,6 +55,15 @@ class Stats(C.Structure):
                 ("t_factor_ms", C.c_double), ("t_cg_ms", C.c_double), ("n_solves", C.c_int),
                 ("resid4_first", C.c_double), ("refine_steps", C.c_int),
                 ("t_refine_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("short_max", C.c_int), ("group_max", C.c_int), ("pair_min", C.c_int),
+                ("count", C.c_int), ("candidates", C.c_int), ("factor_ms", C.c_double),
+                ("default_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -92,12 +102,13 @@ _SIGS = {
     "hykkt_set_options": ([_vp, C.POINTER(Options)], C.c_int),
     "hykkt_get_options": ([_vp, C.POINTER(Options)], C.c_int),
     "hykkt_get_stats": ([_vp, C.POINTER(Stats)], C.c_int),
+    "hykkt_get_tuning": ([_vp, C.POINTER(Tuning)], C.c_int),
     "hykkt_get_solution": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_get_solution_device": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_create_batch": ([C.c_double, C.c_int, C.POINTER(_vp)], C.c_int),
     "hykkt_batch_capacity": ([_vp, _ip], C.c_int),
     "hykkt_bind_system": ([_vp, C.c_int, C.c_double] + [_vp] * 9, C.c_int),
-    "hykkt_execute_batch": ([_vp, C.c_int, _ip], C.c_int),
+    "hykkt_execute_batch": ([_vp, C.c_int, _vp], C.c_int),
     "hykkt_get_stats_k": ([_vp, C.c_int, C.POINTER(Stats)], C.c_int),
     "hykkt_get_solution_k": ([_vp, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_get_solution_device_k": ([_vp, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
@@ -280,6 +291,12 @@ class HykktSolver:
         _check(lib().hykkt_get_stats(self._h, C.byref(st)))
         return st.as_dict()
 
+    def tuning(self) -> dict:
+        """The launch shapes the autotuner chose for the current pattern."""
+        t = Tuning()
+        _check(lib().hykkt_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
     def profile(self, on=True):
         """Start (and clear) or stop per-kernel timing with HIP events."""
         _check(lib().hykkt_profile_enable(self._h, int(bool(on))))
@@ -450,9 +467,8 @@ def ruiz_scale(its, H, Jt, J, rhs1, rhs2):
     r1, r2 = _f64(rhs1).copy(), _f64(rhs2).copy()
     n, m2 = len(hrp) - 1, len(jrp) - 1
     md = np.zeros(n + m2)
-    _check(lib().hykkt_ruiz_scale(int(its), n, m2, _p(hrp), _p(hci), _p(hv), len(hv), _p(trp),
-                                  _p(tci), _p(tv), _p(jrp), _p(jci), _p(jv), len(jv), _p(r1),
-                                  _p(r2), _p(md)))
+    _check(lib().hykkt_ruiz_scale(int(its), n, m2, _p(hrp), _p(hci), _p(hv), len(hv), _p(trp), _p(tci), _p(tv),
+              _p(jrp), _p(jci), _p(jv), len(jv), _p(r1), _p(r2), _p(md)))
     return hv, tv, jv, r1, r2, md
 
 

@@ -76,6 +76,7 @@ int hykkt_default_options(hykkt_options* o)
     o->verbose = 0;
     o->refine_max = 3;
     o->refine_tol = 9e-9;   // the 1e-8 accuracy target less a 10% margin
+    o->autotune = 1;
     return HYKKT_OK;
   });
 }
@@ -204,6 +205,16 @@ int hykkt_set_options(hykkt_solver s, const hykkt_options* o)
                       o->refine_max >= 0,
                   HYKKT_E_ARG, "negative iteration count");
     s->impl.opts = *o;
+    return HYKKT_OK;
+  });
+}
+
+int hykkt_get_tuning(hykkt_solver s, hykkt_tuning* t)
+{
+  return guard([&] {
+    NEED(s);
+    NEED(t);
+    *t = s->impl.tuning();
     return HYKKT_OK;
   });
 }

@@ -22,17 +22,7 @@ namespace hykkt {
 namespace {
 constexpr int NB = 128;   // panel step, MFMA tile edge, solve block
 constexpr int SR = 32;    // rows of a short tile
-// A step whose TRSM has at most this many 128-row tiles runs its TRSM and
-// next-column update in short tiles: four times the workgroups, each with a
-// quarter of the work, for launches that sit on the critical path.
-constexpr int SHORT_MAX = 128;
-// Up to GMAX consecutive steps share one trailing update with K = G * NB
-// (each C tile read and written once for the group) while that update has at
-// least PAIR_MIN * G^2 / 4 tiles of 128 x 128 beyond the G block columns next
-// in line: a group's inner steps update their next block column with K up to
-// (G-1) NB on the critical path, which only a big trailing update repays.
-constexpr int PAIR_MIN = 512;
-constexpr int GMAX = 8;
+constexpr int GMAX = 8;   // most steps one grouped trailing update can cover
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
@@ -42,6 +32,14 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming));
   pl_ = chol_analyze(a, opt);
+  if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) {
+    tun_.pair_min = std::max(1, std::atoi(pm));
+    pinned_ = true;
+  }
+  if (const char* gm = std::getenv("HYKKT_GROUP_MAX")) {
+    tun_.group_max = std::clamp(std::atoi(gm), 1, GMAX);
+    pinned_ = true;
+  }
   {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -102,6 +100,14 @@ CholEngine::Sizes CholEngine::sizes() const
                at_least1(pl_.n), at_least1(n_ubuf_), at_least1(n_bpart_)};
 }
 
+void CholEngine::set_tuning(const Tuning& t, int count)
+{
+  HIPCHK(hipStreamSynchronize(st_));   // the task lists of running launches are replaced
+  tun_ = t;
+  tcount_ = std::max(count, 1);
+  build_tasks();
+}
+
 void CholEngine::bind(const Storage& s, int64_t vs, int cap)
 {
   cd_.L = s.L;
@@ -150,10 +156,19 @@ CholEngine::~CholEngine()
 
 void CholEngine::build_tasks()
 {
-  int pair_min = PAIR_MIN;
-  if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) pair_min = std::max(1, std::atoi(pm));
-  int gmax = GMAX;
-  if (const char* gm = std::getenv("HYKKT_GROUP_MAX")) gmax = std::clamp(std::atoi(gm), 1, GMAX);
+  // Launch shapes (see Tuning), counted over the tcount_ systems a launch
+  // covers: a step whose TRSM has at most short_max tiles of 128 rows runs
+  // its TRSM and next-column update in 32-row tiles (four times the
+  // workgroups, each with a quarter of the work, for launches on the critical
+  // path); up to group_max consecutive steps share one trailing update with
+  // K = G * NB (each C tile read and written once for the group) while that
+  // update has at least pair_min * G^2 / 4 tiles of 128 x 128 beyond the G
+  // block columns next in line (a group's inner steps update their next
+  // block column with K up to (G-1) NB on the critical path, which only a big
+  // trailing update repays).
+  const int64_t tc = std::max(tcount_, 1);
+  const int pair_min = std::max(1, tun_.pair_min);
+  const int gmax = std::clamp(tun_.group_max, 1, GMAX);
   const int nlev = static_cast<int>(pl_.level_ptr.size()) - 1;
   lv_.assign(nlev, Level{});
   std::vector<int> eap, eacol, eacptr{0}, eacc, eaccol;
@@ -225,7 +240,7 @@ void CholEngine::build_tasks()
           const int rt = cdiv(f - cs, NB), ct = cdiv(nc - cs, NB);
           for (int tc = g; tc < ct; tc++) rest += rt - tc;
         }
-        if (rest >= static_cast<int64_t>(pair_min) * g * g / 4) G = g;
+        if (rest * tc >= static_cast<int64_t>(pair_min) * g * g / 4) G = g;
       }
       for (int i = 0; i < G; i++) {
         L.gpos[j + i] = i;
@@ -244,7 +259,7 @@ void CholEngine::build_tasks()
         const int nc = pl_.nc(s), f = nc + pl_.nr(s), k0 = j * NB;
         if (k0 < nc) full_tiles += cdiv(f - (k0 + std::min(NB, nc - k0)), NB);
       }
-      const bool shrt = full_tiles <= SHORT_MAX;
+      const bool shrt = full_tiles * tc <= tun_.short_max;
       const int rows_unit = shrt ? SR : NB;
       L.trsm[j].short_rows = shrt;
       L.upd_next[j].short_rows = shrt && !(last && G > 1);

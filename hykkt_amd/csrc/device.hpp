@@ -74,6 +74,14 @@ class CholEngine {
     double *L, *U, *dinv, *y, *ubuf, *bpart;
     int* pivot;
   };
+  // Launch shapes of the numeric factorization (tile counts are over all the
+  // systems of a launch).  Every choice gives the same bits: a C entry takes
+  // the same MFMA chain in the same K order whatever the tile or group.
+  struct Tuning {
+    int short_max = 128;   // TRSM tiles of 128 rows up to which a step uses 32-row tiles
+    int group_max = 8;     // most steps sharing one trailing update
+    int pair_min = 512;    // tiles a group of G steps needs: pair_min * G^2 / 4
+  };
   // `a`: full n x n CSR whose lower triangle is factored.
   CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st);
   ~CholEngine();
@@ -87,6 +95,11 @@ class CholEngine {
   // Allocate and bind storage for `cap` systems.
   void own(int cap);
   int64_t stride() const { return vs_; }
+  // Rebuild the task lists with launch shapes t for launches of `count` systems.
+  void set_tuning(const Tuning& t, int count);
+  const Tuning& tuning() const { return tun_; }
+  // Launch shapes fixed by HYKKT_PAIR_MIN / HYKKT_GROUP_MAX (not autotuned).
+  bool tuning_pinned() const { return pinned_; }
   double* L() { return cd_.L; }
   // Zero the factor storage (before assembly of the matrix entries).
   void zero_L(const dev::Batch& b);
@@ -137,6 +150,9 @@ class CholEngine {
   hipStream_t st_;
   int64_t vs_ = 0;   // per-system stride of the bound storage
   int cap_ = 0;
+  Tuning tun_;
+  bool pinned_ = false;
+  int tcount_ = 1;   // systems per launch the task lists are shaped for
   DevBuf<double> own_;   // owned storage (own)
   DevBuf<int> own_piv_;
   hipStream_t st2_ = nullptr;   // look-ahead: the rest of each trailing update

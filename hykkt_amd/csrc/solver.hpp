@@ -55,6 +55,7 @@ class Solver {
   std::vector<int> execute_batch(int count);
   int capacity() const { return cap_; }
   const hykkt_stats& stats_of(int k) const;
+  const hykkt_tuning& tuning() const { return tuning_; }
   void get_solution(int k, double* dx, double* ds, double* dyc, double* dyd);
   void get_solution(double* dx, double* ds, double* dyc, double* dyd)
   {
@@ -80,6 +81,9 @@ class Solver {
   std::vector<int> run(int count, bool from_setters);
   // H_gamma from the Ruiz-scaled blocks into the factor storage (+ delta).
   void assemble_hgamma(const dev::Batch& b);
+  // Time the numeric factorization of the batch under candidate launch
+  // shapes and keep the fastest (after formation, before the factorization).
+  void autotune(const dev::Batch& b);
   // Work buffers of one solve pass (r^x permuted, Schur rhs, CG iterate):
   // arena offsets.
   struct Pass {
@@ -106,6 +110,7 @@ class Solver {
   std::vector<SystemIn> sys_in_;
   std::vector<hykkt_stats> st_k_;
   int last_iters_ = 0;   // CG iterations of the last pass (first host check)
+  hykkt_tuning tuning_{};
   // file-path storage (read_matrix_files)
   CsrHost fblk_[4];
   std::vector<double> fvec_[8];

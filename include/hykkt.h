@@ -117,6 +117,9 @@ typedef struct {
                                reference's single pass (default 3; 0 = the reference) */
   double refine_tol;        /* refine while resid4 > refine_tol (default 9e-9); a pass is
                                kept only if it lowers resid4 */
+  int    autotune;          /* 1 (default): at the first solve of a pattern, time the numeric
+                               factorization under several launch shapes (hykkt_tuning) and
+                               keep the fastest; results are bit-identical under all of them */
 } hykkt_options;
 
 typedef struct {
@@ -141,7 +144,21 @@ typedef struct {
   double t_refine_ms;       /* refinement part of t_solve_ms */
 } hykkt_stats;
 
+/* Launch shapes of the numeric factorization the autotuner chose for the
+ * current pattern (no counterpart in the reference). */
+typedef struct {
+  int    short_max;         /* TRSM tiles (of 128 rows, over the systems of a launch) up to which
+                               a step runs its TRSM and next-column update in 32-row tiles */
+  int    group_max;         /* most consecutive steps sharing one trailing update */
+  int    pair_min;          /* trailing-update tiles a group of G steps needs: pair_min*G*G/4 */
+  int    count;             /* systems per launch the shapes were tuned for */
+  int    candidates;        /* shapes timed (0: not tuned, defaults kept) */
+  double factor_ms;         /* factorization time with the chosen shapes */
+  double default_ms;        /* factorization time with the default shapes */
+} hykkt_tuning;
+
 int hykkt_default_options(hykkt_options* o);
+int hykkt_get_tuning(hykkt_solver s, hykkt_tuning* t);
 int hykkt_set_options(hykkt_solver s, const hykkt_options* o);
 int hykkt_get_options(hykkt_solver s, hykkt_options* o);
 int hykkt_get_stats(hykkt_solver s, hykkt_stats* st);

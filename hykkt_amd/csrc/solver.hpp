@@ -103,7 +103,8 @@ class Solver {
   double gamma_;
   int cap_ = 1;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev_[5] = {};
+  hipEvent_t ev_[7] = {};   // 5, 6: around the autotuning runs
+  float tune_ms_ = 0.f;      // device time of the autotuning runs of this pass
   BlockIn blk_[4];
   const double* vecs_[8] = {};
   int vec_n_[8] = {};

@@ -119,7 +119,10 @@ typedef struct {
                                kept only if it lowers resid4 */
   int    autotune;          /* 1 (default): at the first solve of a pattern, time the numeric
                                factorization under several launch shapes (hykkt_tuning) and
-                               keep the fastest; results are bit-identical under all of them */
+                               keep the fastest; results are bit-identical under all of them.
+                               The timing runs are left out of the stats' times and of the
+                               kernel profile.  HYKKT_PAIR_MIN / HYKKT_GROUP_MAX in the
+                               environment fix the shapes instead */
 } hykkt_options;
 
 typedef struct {

@@ -112,10 +112,8 @@ def test_grouped_updates_bit_identical(seq, monkeypatch):
     sy = g.system(19, ds_range=ds_spread_schedule(19, 20))
     out, launches = {}, {}
     for gmax in ("1", "default"):
-        if gmax == "1":
-            monkeypatch.setenv("HYKKT_GROUP_MAX", "1")
-        else:
-            monkeypatch.delenv("HYKKT_GROUP_MAX", raising=False)
+        # the variable fixes the launch shapes (no autotuning)
+        monkeypatch.setenv("HYKKT_GROUP_MAX", "1" if gmax == "1" else "8")
         s = H.HykktSolver(g.gamma)
         try:
             _bind(s, sy)
@@ -130,4 +128,41 @@ def test_grouped_updates_bit_identical(seq, monkeypatch):
           f"{launches['default']} grouped", flush=True)
     assert launches["default"] < launches["1"]
     for a, b in zip(out["1"], out["default"]):
+        assert np.array_equal(a, b)
+
+
+def test_autotuned_shapes_bit_identical(seq, monkeypatch):
+    """The autotuner times the factorization under its candidate launch
+    shapes at the first solve of the pattern and keeps the fastest; the
+    solution has the same bits as with the default shapes, and the tuning
+    runs are left out of the kernel profile and the stats' times."""
+    g, _ = seq
+    monkeypatch.delenv("HYKKT_GROUP_MAX", raising=False)
+    monkeypatch.delenv("HYKKT_PAIR_MIN", raising=False)
+    sy = g.system(19, ds_range=ds_spread_schedule(19, 20))
+    out, kt, st, tun = {}, {}, {}, {}
+    for at in (0, 1):
+        s = H.HykktSolver(g.gamma)
+        try:
+            s.options(autotune=at)
+            _bind(s, sy)
+            s.profile(True)
+            assert s.execute() == 0
+            kt[at] = s.kernel_times()
+            st[at] = s.stats()
+            tun[at] = s.tuning()
+            s.profile(False)
+            out[at] = [np.copy(a) for a in s.solution()]
+        finally:
+            s.close()
+    t = tun[1]
+    print(f"autotune: {t}; factor {st[0]['t_factor_ms']:.2f} ms default, "
+          f"{st[1]['t_factor_ms']:.2f} ms tuned", flush=True)
+    assert tun[0]["candidates"] == 0
+    assert t["candidates"] == 7 and t["count"] == 1
+    assert 0 < t["factor_ms"] <= t["default_ms"]
+    # one factorization's launches in the profile, not eight
+    assert kt[1]["k_potrf_inv"]["launches"] == kt[0]["k_potrf_inv"]["launches"]
+    assert st[1]["t_solve_ms"] < 3 * st[0]["t_solve_ms"]
+    for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)

@@ -128,6 +128,8 @@ _SIGS = {
     "hykkt_profile_get": ([_vp, C.POINTER(KernelTime), C.c_int, _ip], C.c_int),
     "hykkt_ruiz_scale": ([C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                           _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int),
+    "hykkt_ruiz_scale_noalloc": ([C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp,
+                                  _vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp], C.c_int),
     "hykkt_perm_reverse": ([C.c_int, _vp, _vp], C.c_int),
     "hykkt_perm_map_rc": ([C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "hykkt_perm_map_c": ([C.c_int, _vp, _vp, _vp, _vp, _vp], C.c_int),
@@ -458,16 +460,19 @@ def mm_read_rhs(path):
 
 
 # ---------------------------------------------------------------- components
-def ruiz_scale(its, H, Jt, J, rhs1, rhs2):
+def ruiz_scale(its, H, Jt, J, rhs1, rhs2, noalloc=False):
     """RuizClass (RuizClass.cpp:54-98) on the GPU.  H, Jt, J are (rp, ci, v)
-    triples (copied); returns scaled copies (Hv, Jtv, Jv, rhs1, rhs2, max_d)."""
+    triples (copied); returns scaled copies (Hv, Jtv, Jv, rhs1, rhs2, max_d).
+    noalloc: through hykkt_ruiz_scale_noalloc (no device allocation; the
+    library's fixed device scratch)."""
     hrp, hci, hv = _i32(H[0]), _i32(H[1]), _f64(H[2]).copy()
     trp, tci, tv = _i32(Jt[0]), _i32(Jt[1]), _f64(Jt[2]).copy()
     jrp, jci, jv = _i32(J[0]), _i32(J[1]), _f64(J[2]).copy()
     r1, r2 = _f64(rhs1).copy(), _f64(rhs2).copy()
     n, m2 = len(hrp) - 1, len(jrp) - 1
     md = np.zeros(n + m2)
-    _check(lib().hykkt_ruiz_scale(int(its), n, m2, _p(hrp), _p(hci), _p(hv), len(hv), _p(trp), _p(tci), _p(tv),
+    fn = lib().hykkt_ruiz_scale_noalloc if noalloc else lib().hykkt_ruiz_scale
+    _check(fn(int(its), n, m2, _p(hrp), _p(hci), _p(hv), len(hv), _p(trp), _p(tci), _p(tv),
               _p(jrp), _p(jci), _p(jv), len(jv), _p(r1), _p(r2), _p(md)))
     return hv, tv, jv, r1, r2, md
 

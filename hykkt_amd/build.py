@@ -28,6 +28,7 @@ HIP_SOURCES = {
     "kernels_chol.hip": [],
     "kernels_dense.hip": [],
     "kernels_trsv.hip": [],
+    "noalloc.hip": [],
 }
 CXX_SOURCES = ["symbolic.cpp", "amd.cpp", "cholplan.cpp", "mmio.cpp", "chol_engine.cpp",
                "profiler.cpp",

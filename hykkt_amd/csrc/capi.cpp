@@ -526,6 +526,73 @@ int hykkt_ruiz_scale(int its, int n, int m2, const int* h_rp, const int* h_ci, d
   });
 }
 
+int hykkt_ruiz_scale_noalloc(int its, int n, int m2, const int* h_rp, const int* h_ci,
+                             double* h_v, int h_nnz, const int* jt_rp, const int* jt_ci,
+                             double* jt_v, const int* j_rp, const int* j_ci, double* j_v,
+                             int j_nnz, double* rhs1, double* rhs2, double* max_d)
+{
+  return guard([&] {
+    HYKKT_REQUIRE(n >= 0 && m2 >= 0 && h_nnz >= 0 && j_nnz >= 0 && its >= 0, HYKKT_E_ARG,
+                  "negative size");
+    NEED(h_rp);
+    NEED(jt_rp);
+    NEED(j_rp);
+    NEED(rhs1);
+    NEED(rhs2);
+    NEED(max_d);
+    // every array the kernels touch, carved from the device scratch in
+    // 256-byte slots: (host source or null, bytes)
+    struct Part {
+      const void* src;
+      int64_t bytes;
+      void* dev;
+    };
+    Part parts[] = {{h_rp, 4ll * (n + 1)},   {h_ci, 4ll * h_nnz},   {h_v, 8ll * h_nnz},
+                    {jt_rp, 4ll * (n + 1)},  {jt_ci, 4ll * j_nnz},  {jt_v, 8ll * j_nnz},
+                    {j_rp, 4ll * (m2 + 1)},  {j_ci, 4ll * j_nnz},   {j_v, 8ll * j_nnz},
+                    {rhs1, 8ll * n},         {rhs2, 8ll * m2},      {nullptr, 8ll * (n + m2)},
+                    {nullptr, 8ll * (n + m2)}};
+    int64_t need = 0;
+    for (Part& p : parts) need += (std::max<int64_t>(p.bytes, 1) + 255) / 256 * 256;
+    const int64_t cap = static_cast<int64_t>(hykkt::dev::noalloc_scratch_bytes());
+    HYKKT_REQUIRE(need <= cap, HYKKT_E_ARG,
+                  "hykkt_ruiz_scale_noalloc: the system needs " + std::to_string(need) +
+                      " bytes of device scratch; the scratch holds " + std::to_string(cap));
+    char* base = static_cast<char*>(hykkt::dev::noalloc_scratch());
+    int64_t at = 0;
+    for (Part& p : parts) {
+      p.dev = base + at;
+      at += (std::max<int64_t>(p.bytes, 1) + 255) / 256 * 256;
+      if (p.src && p.bytes > 0)
+        HIPCHK(hipMemcpy(p.dev, p.src, static_cast<size_t>(p.bytes), hipMemcpyHostToDevice));
+    }
+    auto I = [&](int k) { return static_cast<int*>(parts[k].dev); };
+    auto D = [&](int k) { return static_cast<double*>(parts[k].dev); };
+    hipStream_t st = nullptr;
+    const hykkt::dev::Batch one;
+    if (its == 0) hykkt::dev::fill(D(11), 1.0, n + m2, one, st);
+    for (int it = 0; it < its; it++) {
+      hykkt::dev::ruiz_rowmax(n, n + m2, I(0), D(2), I(3), nullptr, D(5), I(6), D(8), D(12),
+                              nullptr, one, st);
+      hykkt::dev::ruiz_scale(n, n + m2, I(0), I(1), D(2), I(3), I(4), D(5), I(6), I(7), D(8),
+                             D(12), D(9), D(10), D(11), it == 0, one, st);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    auto down = [](double* dst, const double* src, int64_t cnt) {
+      if (cnt > 0)
+        HIPCHK(hipMemcpy(dst, src, static_cast<size_t>(cnt) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    };
+    down(h_v, D(2), h_nnz);
+    down(jt_v, D(5), j_nnz);
+    down(j_v, D(8), j_nnz);
+    down(rhs1, D(9), n);
+    down(rhs2, D(10), m2);
+    down(max_d, D(11), n + m2);
+    return HYKKT_OK;
+  });
+}
+
 int hykkt_perm_reverse(int n, const int* perm, int* rev)
 {
   return guard([&] {

@@ -167,6 +167,12 @@ void recover_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, in
                const double* dx, const double* ryd, const double* ds_diag, const double* rs,
                double* dsv, double* dyd, const Batch& b, hipStream_t st);
 
+// ---------------------------------------------------------------- scratch
+// The allocation-free entry point's device scratch (noalloc.hip): its size
+// in bytes, and its address on the current device (taken once per device).
+size_t noalloc_scratch_bytes();
+void* noalloc_scratch();
+
 // ---------------------------------------------------------------- Cholesky
 // Offset of (r, c), r >= c, in an n x n lower triangle packed by columns.
 __host__ __device__ inline int64_t packed_lower(int64_t r, int64_t c, int64_t n)

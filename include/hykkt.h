@@ -292,6 +292,21 @@ int hykkt_ruiz_scale(int ruiz_its, int n, int m2,
                      const int* j_rp, const int* j_ci, double* j_v, int j_nnz,
                      double* rhs1, double* rhs2, double* max_d);
 
+/* hykkt_ruiz_scale without any device allocation (no counterpart in the
+ * reference: for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same arguments, semantics and results as hykkt_ruiz_scale; the device
+ * copies of the arrays and the scaling vectors live in a fixed 32 MiB device
+ * scratch that each GPU holds until the process ends.  Systems that need more
+ * (the index and value arrays, rhs, scale and max_d together, each rounded up
+ * to 256 bytes) are refused with HYKKT_E_ARG.  Calls that overlap on one GPU
+ * share that scratch and race. */
+int hykkt_ruiz_scale_noalloc(int ruiz_its, int n, int m2,
+                             const int* h_rp, const int* h_ci, double* h_v, int h_nnz,
+                             const int* jt_rp, const int* jt_ci, double* jt_v,
+                             const int* j_rp, const int* j_ci, double* j_v, int j_nnz,
+                             double* rhs1, double* rhs2, double* max_d);
+
 /* Host permutation maps (permcheck.cu:99-195). */
 int hykkt_perm_reverse(int n, const int* perm, int* rev);
 int hykkt_perm_map_rc(int n, const int* rows, const int* cols, const int* perm, const int* rev,

@@ -78,6 +78,23 @@ def roofline(kt, steps):
             "share_of_step": v["ms"] / steps}
 
 
+def kernel_entry(name, v, steps):
+    """Per-step launches and time of one instrumented kernel, its algorithmic
+    work per launch, and the rate that work runs at against its bound."""
+    launches = max(v["launches"], 1)
+    sec = 1e-3 * v["ms"] / launches
+    e = {"launches_per_step": v["launches"] / steps, "ms_per_step": v["ms"] / steps,
+         "avg_us": 1e6 * sec, "algorithmic_flops_per_launch": v["flops"] / launches,
+         "algorithmic_bytes_per_launch": v["bytes"] / launches}
+    if name in COMPUTE_BOUND:
+        e["tflops"] = v["flops"] / launches / sec / 1e12 if sec > 0 else None
+        e["frac_of_peak"] = e["tflops"] / PEAK_FP64_TFLOPS if sec > 0 else None
+    else:
+        e["gbs"] = v["bytes"] / launches / sec / 1e9 if sec > 0 else None
+        e["frac_of_peak"] = e["gbs"] / PEAK_HBM_GBS if sec > 0 else None
+    return e
+
+
 def cpu_model():
     """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
     model, count = None, 0
@@ -355,12 +372,7 @@ def main():
             "max_resid4": prof_pass["worst"],
             "note": "same K executes again with HIP events around every factor and solve "
                     "launch; roofline and kernels come from this pass"},
-        "kernels": {k: {"launches_per_step": v["launches"] / args.steps,
-                        "ms_per_step": v["ms"] / args.steps,
-                        "avg_us": 1e3 * v["ms"] / max(v["launches"], 1),
-                        "algorithmic_flops_per_launch": v["flops"] / max(v["launches"], 1),
-                        "algorithmic_bytes_per_launch": v["bytes"] / max(v["launches"], 1)}
-                    for k, v in kt.items()},
+        "kernels": {k: kernel_entry(k, v, args.steps) for k, v in kt.items()},
         "served": None,
         "batched": None,
         "cpu_baseline": None,

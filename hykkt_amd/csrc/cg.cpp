@@ -25,11 +25,19 @@ namespace {
 void apply_s(SchurOps& o, const double* v, double alpha, double beta, const double* y_in,
              double* out, const dev::Batch& b, const double* dot_w = nullptr)
 {
+  const double fl = 2.0 * o.nnz * b.count;
+  if (o.prof) o.prof->begin(o.st);
   dev::spmv(o.n, o.jth_rp, o.jth_ci, nullptr, o.jth_v, 1.0, nullptr, v, 0.0, nullptr, o.t1,
             o.jth_blk, o.jth_nblk, b, o.st);
+  if (o.prof)
+    o.prof->end(PK_SPMV, o.st, fl, spmv_bytes(o.n, o.m, o.nnz, false, false, false) * b.count);
   o.chol->solve(o.t1, b);
+  if (o.prof) o.prof->begin(o.st);
   dev::spmv(o.m, o.jh_rp, o.jh_ci, nullptr, o.jh_v, alpha, nullptr, o.t1, beta, y_in, out,
             o.jh_blk, o.jh_nblk, b, o.st, dot_w, dot_w ? o.part : nullptr);
+  if (o.prof)
+    o.prof->end(PK_SPMV, o.st, fl,
+                spmv_bytes(o.m, o.n, o.nnz, false, y_in != nullptr, dot_w != nullptr) * b.count);
 }
 
 }  // namespace

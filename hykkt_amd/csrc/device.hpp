@@ -199,6 +199,8 @@ struct SchurOps {
   double *sc = nullptr, *part = nullptr;   // 8 scalars; partials (max of both reductions)
   int *live = nullptr, *iters = nullptr, *status = nullptr;   // per system
   double* gath = nullptr;   // 2 doubles per system: scalars gathered for the readback
+  Profiler* prof = nullptr;  // optional launch timing of the SpMVs
+  int nnz = 0;               // entries of Jc
 };
 // Partials a SchurOps needs per system.
 int schur_partials(int m, int jh_nblk);

@@ -17,13 +17,15 @@ const char* Profiler::name(int k)
   static const char* const names[PK_COUNT] = {
       "k_dense_nt",           "k_potrf_inv",         "k_extend_add",
       "k_fwd_forest",         "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
-      "k_solve_bwd<256, false>"};
+      "k_solve_bwd<256, false>", "k_spmv",             "k_spgemm_values",
+      "k_spgemm_scatter"};
   return (k >= 0 && k < PK_COUNT) ? names[k] : "";
 }
 
 void Profiler::enable(bool on)
 {
   on_ = on;
+  paused_ = false;
   for (auto& t : tot_) t = Total{};
   for (const Pend& p : pend_) {
     free_.push_back(p.a);
@@ -49,7 +51,7 @@ hipEvent_t Profiler::get()
 
 void Profiler::begin(hipStream_t st)
 {
-  if (!on_) return;
+  if (!on()) return;
   HYKKT_REQUIRE(open_ == nullptr, HYKKT_E_STATE, "profiler: nested begin");
   open_ = get();
   HIPCHK(hipEventRecord(open_, st));
@@ -57,7 +59,7 @@ void Profiler::begin(hipStream_t st)
 
 void Profiler::end(int kind, hipStream_t st, double flops, double bytes)
 {
-  if (!on_) return;
+  if (!on()) return;
   HYKKT_REQUIRE(open_ != nullptr, HYKKT_E_STATE, "profiler: end without begin");
   hipEvent_t b = get();
   HIPCHK(hipEventRecord(b, st));

@@ -20,8 +20,21 @@ enum ProfKind {
   PK_SOLVE_FWD_TOP,
   PK_SOLVE_BWD_TOP,
   PK_SOLVE_BWD_FOREST,
+  PK_SPMV,
+  PK_SPGEMM_VALUES,
+  PK_SPGEMM_SCATTER,
   PK_COUNT
 };
+
+// Algorithmic HBM bytes of one y = alpha A x + beta y_in over a CSR A (rows
+// x cols, nnz entries): row pointers, column indices and values read once
+// (and the value map, when the values are read through one), x and y_in
+// once, y written once; with a fused dot, its other operand read once.
+inline double spmv_bytes(int rows, int cols, double nnz, bool map, bool y_in, bool dot)
+{
+  return 4.0 * (rows + 1) + (map ? 16.0 : 12.0) * nnz + 8.0 * cols +
+         8.0 * rows * (1 + (y_in ? 1 : 0) + (dot ? 1 : 0));
+}
 
 class Profiler {
  public:
@@ -34,7 +47,9 @@ class Profiler {
   Profiler& operator=(const Profiler&) = delete;
   ~Profiler();
   void enable(bool on);   // also clears the totals
-  bool on() const { return on_; }
+  bool on() const { return on_ && !paused_; }
+  // Leave the launches between pause(true) and pause(false) out.
+  void pause(bool p) { paused_ = p; }
   void begin(hipStream_t st);
   void end(int kind, hipStream_t st, double flops, double bytes);
   void collect();
@@ -49,6 +64,7 @@ class Profiler {
   };
   hipEvent_t get();
   bool on_ = false;
+  bool paused_ = false;
   hipEvent_t open_ = nullptr;
   std::vector<Pend> pend_;
   std::vector<hipEvent_t> free_, all_;

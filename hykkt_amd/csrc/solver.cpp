@@ -57,6 +57,8 @@ struct Solver::Sym {
   // row blocks of the LDS-staged SpMVs: Jd^T, Jc^T (internal order), Jc
   DevBuf<int> jdt_blk, jcth_blk, jch_blk;
   int jdt_nblk = 0, jcth_nblk = 0, jch_nblk = 0;
+  // algorithmic work of one system's H~ and H_gamma value kernels (profiler)
+  double ht_flops = 0, ht_bytes = 0, hg_flops = 0, hg_bytes = 0;
   // per-system storage for `cap` systems, and its flags and staging
   Slots z;
   int cap = 0;
@@ -336,6 +338,21 @@ void Solver::analyze()
   upload_or_dummy(S->hg_ta, S->hgp.ta);
   upload_or_dummy(S->hg_tb, S->hgp.tb);
   upload_or_dummy(S->ent_src, ent);
+  {
+    // E = D + alpha A B with the pattern reused: each value of A, B, D and E
+    // moved once, E's indices and row pointers once (the scatter also reads
+    // each entry's 64-bit slot in the factor storage); 2 flops per term and
+    // one per D entry.  H~ = H + Jd^T (Ds Jd), H_gamma = H~ + gamma Jc^T Jc
+    // (its lower triangle).
+    const double ht_terms = static_cast<double>(S->htp.ta.size());
+    double hg_terms = 0;
+    for (int e : ent) hg_terms += S->hgp.term_ptr[e + 1] - S->hgp.term_ptr[e];
+    const double ne = static_cast<double>(S->htp.e.nnz()), ng = static_cast<double>(ent.size());
+    S->ht_flops = 2.0 * ht_terms + S->nnz_h;
+    S->ht_bytes = 8.0 * (2.0 * S->nnz_jd + S->nnz_h + ne) + 4.0 * (ne + n + 1);
+    S->hg_flops = 2.0 * hg_terms + ng;
+    S->hg_bytes = 8.0 * (2.0 * S->nnz_jc + ne + ng) + 4.0 * (ng + n + 1) + 8.0 * ng;
+  }
   S->jch_rp.upload(jch.rp);
   upload_or_dummy(S->jch_ci, jch.ci);
   upload_or_dummy(S->jch_map, jch_map);
@@ -511,9 +528,11 @@ void Solver::assemble_hgamma(const dev::Batch& b)
   const Slots& z = S.z;
   CholEngine& ch = *S.chol;
   ch.zero_L(b);
-  dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, P(z.htil),
-                      S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, P(z.jcs), P(z.jcs), P(z.gam), ch.L(), b,
-                      st_);
+  timed(PK_SPGEMM_SCATTER, S.hg_flops, S.hg_bytes, b, [&] {
+    dev::spgemm_scatter(ch.n_assembled(), S.ent_src.p, ch.a_dst(), S.hg_dsrc.p, P(z.htil),
+                        S.hg_tp.p, S.hg_ta.p, S.hg_tb.p, P(z.jcs), P(z.jcs), P(z.gam), ch.L(), b,
+                        st_);
+  });
   ch.shift_diag(opts.delta, b);   // add_diag(delta) hook (HykktSolver.cpp:392)
 }
 
@@ -549,7 +568,7 @@ void Solver::autotune(const dev::Batch& b)
                     T{128, 8, 256}, T{128, 8, 2048}, T{128, 1, 512}};
   hipEvent_t e0 = ev_[3], e1 = ev_[4];   // free until the solve phase
   // the tuning runs are neither profiled nor counted in the pass's times
-  ch.set_profiler(nullptr);
+  prof.pause(true);
   HIPCHK(hipEventRecord(ev_[5], st_));
   auto run_one = [&](const T& t) {
     ch.set_tuning(t, b.count);
@@ -578,7 +597,7 @@ void Solver::autotune(const dev::Batch& b)
   HIPCHK(hipEventRecord(ev_[6], st_));
   HIPCHK(hipEventSynchronize(ev_[6]));
   HIPCHK(hipEventElapsedTime(&tune_ms_, ev_[5], ev_[6]));
-  ch.set_profiler(&prof);
+  prof.pause(false);
   tuning_.short_max = cand[bi].short_max;
   tuning_.group_max = cand[bi].group_max;
   tuning_.pair_min = cand[bi].pair_min;
@@ -664,10 +683,14 @@ std::vector<int> Solver::run(int count, bool from_setters)
   if (S.has_jd) {
     dev::row_scale(d, S.jd_rp.p, in[IN_JD], in[IN_DS], in[IN_RYD], in[IN_RS], P(z.jd_vs),
                    P(z.ryd_s), b, st_);
-    dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s), 1.0,
-              in[IN_RX], P(z.rx_til), S.jdt_blk.p, S.jdt_nblk, b, st_);
-    dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, in[IN_H], S.ht_tp.p, S.ht_ta.p, S.ht_tb.p,
-                       in[IN_JD], P(z.jd_vs), 1.0, P(z.htil), b, st_);
+    timed(PK_SPMV, 2.0 * S.nnz_jd, spmv_bytes(n, d, S.nnz_jd, true, true, false), b, [&] {
+      dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s), 1.0,
+                in[IN_RX], P(z.rx_til), S.jdt_blk.p, S.jdt_nblk, b, st_);
+    });
+    timed(PK_SPGEMM_VALUES, S.ht_flops, S.ht_bytes, b, [&] {
+      dev::spgemm_values(S.htp.e.nnz(), S.ht_dsrc.p, in[IN_H], S.ht_tp.p, S.ht_ta.p, S.ht_tb.p,
+                         in[IN_JD], P(z.jd_vs), 1.0, P(z.htil), b, st_);
+    });
   } else {
     dev::copy(P(z.rx_til), in[IN_RX], n, b, st_);
     dev::copy(P(z.htil), in[IN_H], S.nnz_h, b, st_);
@@ -761,8 +784,10 @@ std::vector<int> Solver::run(int count, bool from_setters)
     if (S.has_jd) {
       dev::row_scale(d, S.jd_rp.p, in[IN_JD], in[IN_DS], P(z.e_d), P(z.e_s), nullptr, P(z.ryd_s),
                      ba, st_);
-      dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s), 1.0,
-                P(z.e_x), P(z.e_rxt), S.jdt_blk.p, S.jdt_nblk, ba, st_);
+      timed(PK_SPMV, 2.0 * S.nnz_jd, spmv_bytes(n, d, S.nnz_jd, true, true, false), ba, [&] {
+        dev::spmv(n, S.jdt_rp.p, S.jdt_ci.p, S.jdt_map.p, in[IN_JD], 1.0, nullptr, P(z.ryd_s),
+                  1.0, P(z.e_x), P(z.e_rxt), S.jdt_blk.p, S.jdt_nblk, ba, st_);
+      });
     } else {
       dev::copy(P(z.e_rxt), P(z.e_x), n, ba, st_);
     }
@@ -848,12 +873,19 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   const int n = S.n, m = S.m, d = S.d;
   // r^x = r~x + gamma Jc^T ry in the factor's internal order
   dev::gather(P(z.tmp_n), P(rxt), S.phat_d.p, n, b, st_);
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), 0.0, P(z.gam), P(ysc), 1.0,
-            P(z.tmp_n), P(w.rxp), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  const double fl = 2.0 * S.nnz_jc;
+  const double by_t = spmv_bytes(n, m, S.nnz_jc, false, true, false);   // Jc^T x + y
+  const double by_j = spmv_bytes(m, n, S.nnz_jc, false, true, false);   // Jc x + y
+  timed(PK_SPMV, fl, by_t, b, [&] {
+    dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), 0.0, P(z.gam), P(ysc), 1.0,
+              P(z.tmp_n), P(w.rxp), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  });
   dev::copy(P(z.t1), P(w.rxp), n, b, st_);
   S.chol->solve(P(z.t1), b);
-  dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, P(z.jch_v), 1.0, nullptr, P(z.t1), -1.0, P(ysc),
-            P(w.schur), S.jch_blk.p, S.jch_nblk, b, st_);
+  timed(PK_SPMV, fl, by_j, b, [&] {
+    dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, P(z.jch_v), 1.0, nullptr, P(z.t1), -1.0,
+              P(ysc), P(w.schur), S.jch_blk.p, S.jch_nblk, b, st_);
+  });
 
   SchurOps ops;
   ops.m = m;
@@ -882,6 +914,8 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   ops.iters = S.flag(FL_CG_IT);
   ops.status = S.flag(FL_CG_ST);
   ops.gath = S.gath.p;
+  ops.prof = &prof;
+  ops.nnz = S.nnz_jc;
   if (x0) dev::copy(P(w.cx), P(z.in[IN_Y0]), m, b, st_);
   else dev::fill(P(w.cx), 0.0, m, b, st_);
   // the host looks at the live flags first after as many iterations as the
@@ -892,8 +926,10 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
       schur_cg_run(ops, P(w.schur), opts.cg_tol, opts.cg_itmax, b, !x0, chunk, chunk_next);
   if (after_cg) HIPCHK(hipEventRecord(after_cg, st_));
 
-  dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), -1.0, nullptr, P(w.cx), 1.0,
-            P(w.rxp), P(z.t1), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  timed(PK_SPMV, fl, by_t, b, [&] {
+    dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), -1.0, nullptr, P(w.cx), 1.0,
+              P(w.rxp), P(z.t1), S.jcth_blk.p, S.jcth_nblk, b, st_);
+  });
   S.chol->solve(P(z.t1), b);
   dev::gather_scale(P(dx), P(z.t1), S.ipos_d.p, P(z.max_d), n, b, st_);
   dev::copy(P(dyc), P(w.cx), m, b, st_);

@@ -99,6 +99,15 @@ class Solver {
   // The res slots of `count` systems to the host (synchronizes).
   std::vector<double> read_res(int count);
   double* P(int64_t off) const;   // system 0's address of an arena slot
+  // Run the launch in fn bracketed for the profiler, charged with the work
+  // of the b.count systems it covers.
+  template <class F>
+  void timed(int kind, double flops, double bytes, const dev::Batch& b, F&& fn)
+  {
+    prof.begin(st_);
+    fn();
+    prof.end(kind, st_, flops * b.count, bytes * b.count);
+  }
 
   double gamma_;
   int cap_ = 1;

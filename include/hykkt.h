@@ -265,6 +265,11 @@ int hykkt_device_mem_info(unsigned long long* free_bytes, unsigned long long* to
  *   k_potrf_inv   128-column diagonal blocks (b^3/3)
  *   k_extend_add  update-matrix assembly (1 flop, 24 bytes per entry)
  *   k_solve_fwd / k_solve_bwd  one sweep each (2*nnz(L) flops, 8*nnz(L) bytes)
+ *   k_spmv        CSR products with Jc, Jc^T, Jd^T (2*nnz flops; indices, values,
+ *                 x and y each moved once)
+ *   k_spgemm_values / k_spgemm_scatter  H~ and H_gamma values (2 flops and 24
+ *                 bytes per term, plus each output entry's indices, D term and
+ *                 store)
  * hykkt_profile_enable clears the totals; hykkt_profile_get writes up to
  * `capacity` entries, one per kernel launched since, and sets *count to the
  * number of those kernels. */

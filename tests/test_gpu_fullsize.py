@@ -95,6 +95,13 @@ def test_kernel_timing_accounting(seq):
         assert kt[fwd]["launches"] == kt[bwd]["launches"] >= 3
     fl = kt["k_dense_nt"]["flops"] + kt["k_potrf_inv"]["flops"]
     assert 0.999 <= fl / st["factor_flops"] <= 1.2, fl / st["factor_flops"]
+    # formation: one H~ and one H_gamma value launch; the SpMVs: J_d^T once
+    # per pass, J_c^T and J_c around the CG and twice per Schur apply
+    assert kt["k_spgemm_values"]["launches"] == kt["k_spgemm_scatter"]["launches"] == 1
+    passes = 1 + st["refine_steps"]
+    assert kt["k_spmv"]["launches"] >= passes * 4 + 2 * st["cg_iters"]
+    assert kt["k_spgemm_values"]["bytes"] > 12 * st["nnz_htil"]
+    assert kt["k_spgemm_scatter"]["bytes"] > 20 * st["nnz_hgam"] / 2
     # event-timed kernels fit inside the solve's own wall span: each kernel's
     # sum, and the total, which may count the look-ahead stream's updates
     # twice over the time they overlap the main stream

@@ -11,8 +11,12 @@ requests are rejected.  Request values (perturbed H, Jc, Jd values, Ds,
 right-hand sides) come from a pool per pattern resident in HBM, so the
 request path copies nothing from the host.
 
-Serving.  Requests wait in their pattern's FIFO (bounded: arrivals beyond it
-are rejected and counted).  Each slot has a host worker: whenever the slot is
+Admission.  A pattern admits at most `queue_batches` batches' worth of
+waiting requests (queue_batches x its slot capacity); arrivals beyond that
+are rejected and counted, so under overload the queues, and the latency of
+an admitted request, stay bounded.  The plain path admits by the same rule.
+
+Serving.  Requests wait in their pattern's FIFO.  Each slot has a host worker: whenever the slot is
 free it takes every request of its pattern that has arrived, up to the slot's
 capacity, and solves them as one batch -- every kernel launch covers all of
 them.  Requests that arrive while a batch is on the GPU join the next one, so
@@ -156,11 +160,11 @@ class Server:
     `hbm_budget` bytes; see the module docstring.  capacity: systems per batch
     slot (a function of the bus count, default_capacity by default)."""
 
-    def __init__(self, patterns, hbm_budget: float, capacity=None, queue_limit: int = 4096,
+    def __init__(self, patterns, hbm_budget: float, capacity=None, queue_batches: int = 4,
                  keep_solutions: bool = False):
         self.patterns = {p.pid: p for p in patterns}
         self.keep = keep_solutions
-        self.queue_limit = queue_limit
+        self.queue_batches = queue_batches
         self.slots = {}      # pid -> BatchSolver
         self.plain = {}      # pid -> HykktSolver (the drop-in path)
         self.failures = []   # messages of solves the library reported as failed
@@ -279,6 +283,9 @@ class Server:
         results, lock = [], threading.Lock()
         stop = threading.Event()
         rejected = [0]
+        # admitted requests of each pattern not yet taken by a worker
+        waiting = {pid: 0 for pid in self.served}
+        limit = {pid: self.queue_batches * self.capacity[pid] for pid in self.served}
         if plain:
             queues = {None: queue.Queue()}
             keys = [None]
@@ -300,6 +307,8 @@ class Server:
                     continue
                 if req is None:
                     return
+                with lock:
+                    waiting[req.pattern] -= 1
                 if late():
                     continue   # past the deadline: pending, not run
                 if key is None:
@@ -314,6 +323,8 @@ class Server:
                         if nxt is None:
                             done = True
                             break
+                        with lock:
+                            waiting[nxt.pattern] -= 1
                         batch.append(nxt)
                     res = self._solve_batch(batch)
                 t = time.perf_counter() - t0
@@ -338,7 +349,11 @@ class Server:
                 break
             arrived += 1
             key = None if plain else req.pattern
-            if key not in queues or queues[key].qsize() >= self.queue_limit:
+            with lock:
+                full = req.pattern not in waiting or waiting[req.pattern] >= limit[req.pattern]
+                if not full:
+                    waiting[req.pattern] += 1
+            if full:
                 rejected[0] += 1
                 continue
             queues[key].put(req)

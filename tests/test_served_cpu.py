@@ -82,8 +82,53 @@ def _fake_server(npat=3, cap=4):
     srv.slots = {i: _FakeBatch() for i in range(npat)}
     srv.capacity = {i: cap for i in range(npat)}
     srv.served = list(range(npat))
-    srv.keep, srv.queue_limit, srv.failures, srv.refused = False, 4096, [], []
+    srv.keep, srv.queue_batches, srv.failures, srv.refused = False, 4, [], []
     return srv
+
+
+class _BlockingSolver(_FakeSolver):
+    """A solver whose solves wait for `gate`: requests pile up meanwhile."""
+
+    def __init__(self, gate):
+        super().__init__()
+        self.gate = gate
+
+    def execute(self):
+        self.gate.wait(5.0)
+        return super().execute()
+
+
+class _BlockingBatch(_FakeBatch):
+    def __init__(self, gate):
+        super().__init__()
+        self.gate = gate
+
+    def execute_batch(self, count):
+        self.gate.wait(5.0)
+        return super().execute_batch(count)
+
+
+def test_admission_bounds_the_queues():
+    """A pattern admits queue_batches x its capacity waiting requests; the
+    rest are rejected while its worker is busy, in both forms, so the queues
+    (and an admitted request's wait) stay bounded under overload."""
+    import threading
+    from hykkt_amd.served import Request
+    reqs = [Request(i, i % 3, 0, 0.0) for i in range(24)]   # 8 per pattern, all at once
+    for plain in (True, False):
+        gate = threading.Event()
+        srv = _fake_server(cap=2)
+        srv.queue_batches = 1   # 2 waiting requests per pattern
+        srv.plain = {i: _BlockingSolver(gate) for i in range(3)}
+        srv.slots = {i: _BlockingBatch(gate) for i in range(3)}
+        timer = threading.Timer(0.5, gate.set)
+        timer.start()
+        res, rej, pend, wall = srv.run(reqs, plain=plain)
+        timer.join()
+        assert pend == 0 and len(res) + rej == 24
+        # waiting <= 2 per pattern, plus what the busy workers had taken
+        in_flight = 1 if plain else 3 * 2
+        assert 24 - (6 + in_flight) <= rej <= 24 - 6, (plain, rej)
 
 
 def test_workers_survive_failed_requests():

@@ -56,6 +56,9 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
     HIPCHK(hipMemcpyAsync(o.live, bt.mask, sizeof(int) * cnt, hipMemcpyDeviceToDevice, o.st));
   else
     dev::fill_i(o.live, 1, cnt, o.st);
+  // gam and gam1 of every system start at 0, so one that takes no part
+  // reports ||r|| = 0, not a previous pass's value
+  dev::fill(o.sc, 0.0, 2, dev::Batch{cnt, bt.vs, nullptr}, o.st);
   dev::fill_i(o.iters, 0, cnt, o.st);
   dev::fill_i(o.status, 0, cnt, o.st);
   // setup(): p = s = b (:89-100); r = b - S x0 (:104-122).  With x0 = 0 the

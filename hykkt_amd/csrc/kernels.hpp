@@ -167,6 +167,11 @@ void recover_d(int d, const int* jd_rp, const int* jd_ci, const double* jd_v, in
                const double* dx, const double* ryd, const double* ds_diag, const double* rs,
                double* dsv, double* dyd, const Batch& b, hipStream_t st);
 
+// ok[s] = (system s's factorization met no failing pivot); a system that
+// did gets a zero solution (dx n, dsv d, dyc m, dyd d).
+void pivot_split(const int* pivot, int* ok, int n, int m, int d, double* dx, double* dsv,
+                 double* dyc, double* dyd, const Batch& b, hipStream_t st);
+
 // ---------------------------------------------------------------- scratch
 // The allocation-free entry point's device scratch (noalloc.hip): its size
 // in bytes, and its address on the current device (taken once per device).

@@ -635,6 +635,33 @@ __global__ void k_recover_d(int d, const int* __restrict__ rp, const int* __rest
   }
 }
 
+// ok[s] = 1 when system s's factorization met no failing pivot, else 0,
+// and then its solution (dx, dsv, dyc, dyd) is zero: such a system takes no
+// part in the solve phase.
+__global__ void k_pivot_split(const int* __restrict__ pivot, int* __restrict__ ok, int n, int m,
+                              int d, double* __restrict__ dx, double* __restrict__ dsv,
+                              double* __restrict__ dyc, double* __restrict__ dyd, Batch b)
+{
+  const int sy = sys_of(b);
+  if (sy < 0) return;
+  const bool failed = pivot[sy] != INT_MAX;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ok[sy] = failed ? 0 : 1;
+  if (!failed) return;
+  dx = at(dx, sy, b);
+  dsv = at(dsv, sy, b);
+  dyc = at(dyc, sy, b);
+  dyd = at(dyd, sy, b);
+  const int len = max(n, max(m, d));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) {
+    if (i < n) dx[i] = 0.0;
+    if (i < m) dyc[i] = 0.0;
+    if (i < d) {
+      dsv[i] = 0.0;
+      dyd[i] = 0.0;
+    }
+  }
+}
+
 }  // namespace
 
 // =================================================================== launchers
@@ -885,6 +912,15 @@ void resid_rows_d(int d, const int* rp, const int* ci, const double* v, int has_
   if (d <= 0 || b.count <= 0) return;
   k_resid_d<<<grid2(d, b), TPB, 0, st>>>(d, rp, ci, v, has_jd, ds, rs, ryd, dx, dsv, dyd, sq2,
                                          sq4, e2, e4, b);
+  LAUNCH_CHECK();
+}
+
+void pivot_split(const int* pivot, int* ok, int n, int m, int d, double* dx, double* dsv,
+                 double* dyc, double* dyd, const Batch& b, hipStream_t st)
+{
+  if (b.count <= 0) return;
+  const int len = std::max(1, std::max(n, std::max(m, d)));
+  k_pivot_split<<<grid2(len, b), TPB, 0, st>>>(pivot, ok, n, m, d, dx, dsv, dyc, dyd, b);
   LAUNCH_CHECK();
 }
 

@@ -34,7 +34,7 @@ struct Solver::Slots {
 };
 
 // per-system int flags, one array of `cap` each
-enum { FL_PIVOT = 0, FL_ZERO_ROW, FL_LIVE, FL_ACT, FL_KEEP, FL_CG_IT, FL_CG_ST, FL_COUNT };
+enum { FL_PIVOT = 0, FL_ZERO_ROW, FL_LIVE, FL_ACT, FL_KEEP, FL_CG_IT, FL_CG_ST, FL_OK, FL_COUNT };
 constexpr int RES_SLOTS = 8;
 
 struct Solver::Sym {
@@ -725,10 +725,16 @@ std::vector<int> Solver::run(int count, bool from_setters)
   dev::gather(P(z.jcth_v), P(z.jcs), S.jcth_map.p, S.nnz_jc, b, st_);
 
   // ---- first pass: the reference's solve (HykktSolver.cpp:404-457) ----
+  // A system whose factorization met a failing pivot (reported with status 1,
+  // as the reference's) gets a zero solution and skips the solve phase, so a
+  // batch does not run its CG to the iteration limit on a broken factor.
+  dev::pivot_split(S.flag(FL_PIVOT), S.flag(FL_OK), n, m, d, P(z.dx), P(z.dsv), P(z.dyc),
+                   P(z.dyd), b, st_);
+  const dev::Batch bok = dev::masked(b, S.flag(FL_OK));
   // CG start y_host, or zero when none is bound (then no start-residual solve)
   const std::vector<CgResult> cg = solve_phase(Pass{z.rxp, z.schur, z.cx}, z.rx_til, z.ry_s,
                                                z.in[IN_RS], z.in[IN_RYD], any_y0, z.dx, z.dsv,
-                                               z.dyc, z.dyd, b, ev_[2]);
+                                               z.dyc, z.dyd, bok, ev_[2]);
   int it_max = 0;
   for (const CgResult& r : cg) it_max = std::max(it_max, r.iters);
   last_iters_ = it_max;

@@ -164,6 +164,14 @@ def test_batch_with_failing_systems():
     assert rc[0] == 0 and rc[3] == 0
     assert rc[1] == -1           # HYKKT_E_ARG: the zero row
     assert rc[2] == 1 and b.stats_k(2)["pivot"] >= 0
+    # the failed factorization's system skips the solve phase: zero solution,
+    # no CG iterations, relative residual 1
+    st2 = b.stats_k(2)
+    assert st2["cg_iters"] == 0 and st2["refine_steps"] == 0
+    assert abs(st2["resid4"] - 1.0) < 1e-12
+    assert not np.any(np.concatenate(b.solution_k(2)))
+    rc2, st2a, x2a, _ = alone(g, bad_piv)
+    assert rc2 == 1 and st2a["cg_iters"] == 0 and not np.any(x2a)
     rc1, _, _, _ = alone(g, bad_row)
     assert rc1 == -1
     check_same(g, [systems[0]], b, rc[:1])

@@ -76,7 +76,13 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
   // w = S r, delta = w.r, alpha = gam/delta (:124-144)
   apply_s(o, o.r, 1.0, 0.0, nullptr, o.w, live, o.r);
   dev::dot_final(o.jh_nblk, o.part, o.sc, 2, false, true, -1, tol, itmax, fl, live, o.st);
-  std::vector<int> lv(cnt);
+  // host copies: pinned when the caller gave memory for them
+  std::vector<double> own(o.pinned ? 0 : 4 * static_cast<size_t>(cnt));
+  double* hostmem = o.pinned ? o.pinned : own.data();
+  int* lv = reinterpret_cast<int*>(hostmem);            // cnt ints
+  int* it = lv + cnt;                                    // cnt ints
+  int* stt = it + cnt;                                   // cnt ints
+  double* sc = hostmem + 2 * static_cast<size_t>(cnt);   // 2 cnt doubles (after 4 cnt ints)
   int next_check = std::max(1, chunk);
   for (int i = 0; i < itmax; i++) {
     dev::cg_update_dot(m, o.sc, o.p, o.s, o.x, o.r, o.w, o.part, live, o.st);
@@ -85,21 +91,18 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
     apply_s(o, o.r, 1.0, 0.0, nullptr, o.w, live, o.r);
     dev::dot_final(o.jh_nblk, o.part, o.sc, 2, false, true, i, tol, itmax, fl, live, o.st);
     if (i + 1 == next_check && i + 1 < itmax) {
-      HIPCHK(hipMemcpyAsync(lv.data(), o.live, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
+      HIPCHK(hipMemcpyAsync(lv, o.live, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
       HIPCHK(hipStreamSynchronize(o.st));
-      if (std::none_of(lv.begin(), lv.end(), [](int v) { return v != 0; })) break;
+      if (std::none_of(lv, lv + cnt, [](int v) { return v != 0; })) break;
       next_check += std::max(1, chunk_next);
     }
   }
   // per system: iterations, status and ||r|| at exit (gam1, or gam for a
   // system that stopped at the start)
-  std::vector<int> it(cnt), stt(cnt);
-  std::vector<double> sc(2 * static_cast<size_t>(cnt));
   dev::collect(o.gath, o.sc, 2, dev::Batch{cnt, bt.vs, nullptr}, o.st);
-  HIPCHK(hipMemcpyAsync(it.data(), o.iters, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
-  HIPCHK(hipMemcpyAsync(stt.data(), o.status, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
-  HIPCHK(hipMemcpyAsync(sc.data(), o.gath, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost,
-                        o.st));
+  HIPCHK(hipMemcpyAsync(it, o.iters, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
+  HIPCHK(hipMemcpyAsync(stt, o.status, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
+  HIPCHK(hipMemcpyAsync(sc, o.gath, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost, o.st));
   HIPCHK(hipStreamSynchronize(o.st));
   for (int k = 0; k < cnt; k++) {
     res[k].iters = it[k];

@@ -594,6 +594,8 @@ std::vector<int> CholEngine::pivots(int count)
   return h;
 }
 
+void CholEngine::clear_solve_error() { dev::fill_i(sd_.err, 0, 1, st_); }
+
 int CholEngine::solve_error()
 {
   int h = 0;

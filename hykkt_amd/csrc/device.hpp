@@ -57,6 +57,30 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory (copies into it are asynchronous).
+template <class T>
+struct HostBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { release(); }
+  void ensure(size_t cnt)
+  {
+    if (cnt <= n) return;
+    release();
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(T), hipHostMallocDefault));
+    n = cnt;
+  }
+  void release()
+  {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
 // Numeric supernodal multifrontal Cholesky on the device for one sparsity
 // pattern (the analysis is done once; factor() and solve() run per solve),
 // for up to `capacity` systems of that pattern at once.  The per-system
@@ -98,6 +122,12 @@ class CholEngine {
   // Rebuild the task lists with launch shapes t for launches of `count` systems.
   void set_tuning(const Tuning& t, int count);
   const Tuning& tuning() const { return tun_; }
+  // Device words the solver reads back with its other per-pass results:
+  // each system's first failing pivot (INT_MAX: none) and the sweeps' wait-
+  // bound error word (0: none; clear_solve_error resets it).
+  const int* pivot_dev() const { return cd_.pivot; }
+  const int* solve_error_dev() const { return sd_.err; }
+  void clear_solve_error();
   // Launch shapes fixed by HYKKT_PAIR_MIN / HYKKT_GROUP_MAX (not autotuned).
   bool tuning_pinned() const { return pinned_; }
   double* L() { return cd_.L; }
@@ -201,6 +231,9 @@ struct SchurOps {
   double* gath = nullptr;   // 2 doubles per system: scalars gathered for the readback
   Profiler* prof = nullptr;  // optional launch timing of the SpMVs
   int nnz = 0;               // entries of Jc
+  // optional pinned host memory of at least 4 doubles per system, for the
+  // live-flag checks and the results (pageable vectors otherwise)
+  double* pinned = nullptr;
 };
 // Partials a SchurOps needs per system.
 int schur_partials(int m, int jh_nblk);

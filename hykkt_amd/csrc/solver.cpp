@@ -67,6 +67,8 @@ struct Solver::Sym {
   DevBuf<double> gath;       // cap * RES_SLOTS: per-system scalars gathered for a readback
   DevBuf<double> gam_dev;    // cap gammas (staging for the copy into the arena)
   DevBuf<const double*> tab; // cap * (IN_COUNT + 1) caller pointers for copy_in
+  HostBuf<double> readback;  // pinned: per-pass results copied back in one round trip
+  HostBuf<double> cg_host;   // pinned: the CG's live flags and results
   std::vector<const double*> tab_h;
   int* flag(int which) { return flags.p + static_cast<int64_t>(which) * cap; }
 };
@@ -469,6 +471,7 @@ void Solver::analyze()
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   last_iters_ = 0;
+  last_ref_iters_ = 0;
   sym_ = std::move(S);
 }
 
@@ -734,7 +737,7 @@ std::vector<int> Solver::run(int count, bool from_setters)
   // CG start y_host, or zero when none is bound (then no start-residual solve)
   const std::vector<CgResult> cg = solve_phase(Pass{z.rxp, z.schur, z.cx}, z.rx_til, z.ry_s,
                                                z.in[IN_RS], z.in[IN_RYD], any_y0, z.dx, z.dsv,
-                                               z.dyc, z.dyd, bok, ev_[2]);
+                                               z.dyc, z.dyd, bok, ev_[2], last_iters_);
   int it_max = 0;
   for (const CgResult& r : cg) it_max = std::max(it_max, r.iters);
   last_iters_ = it_max;
@@ -742,13 +745,8 @@ std::vector<int> Solver::run(int count, bool from_setters)
   // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
   residual(z.dx, z.dsv, z.dyc, z.dyd, true, b);
   HIPCHK(hipEventRecord(ev_[3], st_));
-  std::vector<int> zrow(count), keep(count), act(count);
-  HIPCHK(hipMemcpyAsync(zrow.data(), S.flag(FL_ZERO_ROW), sizeof(int) * count,
-                        hipMemcpyDeviceToHost, st_));
-  std::vector<double> hr = read_res(count);
-  const std::vector<int> piv = ch.pivots(count);
-  HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
-                "triangular-solve sweep exceeded its wait bound (results invalid)");
+  std::vector<int> zrow(count), keep(count), act(count), piv(count);
+  std::vector<double> hr = read_res(count, &zrow, &piv);
   std::vector<int> rc(count, HYKKT_OK);
   std::vector<double> nb(count);
   for (int k = 0; k < count; k++) {
@@ -800,16 +798,22 @@ std::vector<int> Solver::run(int count, bool from_setters)
     dev::vec_scale(P(z.e_rxt), P(z.max_d), n, ba, st_);
     dev::copy(P(z.e_ysc), P(z.e_y), m, ba, st_);
     dev::vec_scale(P(z.e_ysc), P(z.max_d) + n, m, ba, st_);
-    solve_phase(corr, z.e_rxt, z.e_ysc, z.e_s, z.e_d, false, z.c_dx, z.c_dsv, z.c_dyc, z.c_dyd, ba,
-                nullptr);
+    // the correction's right-hand side is small: its CG usually stops within
+    // an iteration or two, so its first look at the live flags comes from
+    // the last refinement pass, not the first pass
+    const std::vector<CgResult> cgr =
+        solve_phase(corr, z.e_rxt, z.e_ysc, z.e_s, z.e_d, false, z.c_dx, z.c_dsv, z.c_dyc,
+                    z.c_dyd, ba, nullptr, last_ref_iters_);
+    int rit = 0;
+    for (int k = 0; k < count; k++)
+      if (act[k]) rit = std::max(rit, cgr[k].iters);
+    last_ref_iters_ = rit;
     dev::add(P(z.c_dx), P(z.c_dx), P(z.dx), n, ba, st_);
     dev::add(P(z.c_dsv), P(z.c_dsv), P(z.dsv), d, ba, st_);
     dev::add(P(z.c_dyc), P(z.c_dyc), P(z.dyc), m, ba, st_);
     dev::add(P(z.c_dyd), P(z.c_dyd), P(z.dyd), d, ba, st_);
     residual(z.c_dx, z.c_dsv, z.c_dyc, z.c_dyd, true, ba);
     hr = read_res(count);
-    HYKKT_REQUIRE(!ch.solve_error(), HYKKT_E_STATE,
-                  "triangular-solve sweep exceeded its wait bound (results invalid)");
     int nkeep = 0;
     for (int k = 0; k < count; k++) {
       keep[k] = 0;
@@ -872,7 +876,7 @@ std::vector<int> Solver::run(int count, bool from_setters)
 std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ysc, int64_t rs,
                                           int64_t ryd, bool x0, int64_t dx, int64_t dsv,
                                           int64_t dyc, int64_t dyd, const dev::Batch& b,
-                                          hipEvent_t after_cg)
+                                          hipEvent_t after_cg, int iters_hint)
 {
   Sym& S = *sym_;
   const Slots& z = S.z;
@@ -922,11 +926,13 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   ops.gath = S.gath.p;
   ops.prof = &prof;
   ops.nnz = S.nnz_jc;
+  S.cg_host.ensure(4 * static_cast<size_t>(b.count));
+  ops.pinned = S.cg_host.p;
   if (x0) dev::copy(P(w.cx), P(z.in[IN_Y0]), m, b, st_);
   else dev::fill(P(w.cx), 0.0, m, b, st_);
   // the host looks at the live flags first after as many iterations as the
   // last pass took (or every cg_check_every iterations when that is set)
-  const int chunk = opts.cg_check_every > 0 ? opts.cg_check_every : std::max(1, last_iters_);
+  const int chunk = opts.cg_check_every > 0 ? opts.cg_check_every : std::max(1, iters_hint);
   const int chunk_next = opts.cg_check_every > 0 ? opts.cg_check_every : 1;
   std::vector<CgResult> res =
       schur_cg_run(ops, P(w.schur), opts.cg_tol, opts.cg_itmax, b, !x0, chunk, chunk_next);
@@ -976,15 +982,46 @@ void Solver::residual(int64_t dx, int64_t dsv, int64_t dyc, int64_t dyd, bool ro
   dev::dot(d, in[IN_RYD], in[IN_RYD], part, r + 7, b, st_);
 }
 
-std::vector<double> Solver::read_res(int count)
+std::vector<double> Solver::read_res(int count, std::vector<int>* zrow, std::vector<int>* piv)
 {
   Sym& S = *sym_;
-  std::vector<double> h(static_cast<size_t>(RES_SLOTS) * count);
+  CholEngine& ch = *S.chol;
+  // one host round trip: the residual slots, the sweeps' error word and,
+  // with zrow / piv, each system's zero-row flag and first failing pivot,
+  // copied into pinned memory one after the other
+  const size_t nres = static_cast<size_t>(RES_SLOTS) * count;
+  const size_t words = 1 + (zrow ? count : 0) + (piv ? count : 0);
+  S.readback.ensure(nres + (words + 1) / 2);
+  double* hres = S.readback.p;
+  int* hint = reinterpret_cast<int*>(S.readback.p + nres);
   dev::collect(S.gath.p, P(S.z.res), RES_SLOTS, dev::Batch{count, S.z.stride, nullptr}, st_);
-  HIPCHK(hipMemcpyAsync(h.data(), S.gath.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
-                        st_));
+  HIPCHK(hipMemcpyAsync(hres, S.gath.p, sizeof(double) * nres, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(hint, ch.solve_error_dev(), sizeof(int), hipMemcpyDeviceToHost, st_));
+  int at = 1;
+  if (zrow) {
+    HIPCHK(hipMemcpyAsync(hint + at, S.flag(FL_ZERO_ROW), sizeof(int) * count,
+                          hipMemcpyDeviceToHost, st_));
+    at += count;
+  }
+  if (piv) {
+    HIPCHK(hipMemcpyAsync(hint + at, ch.pivot_dev(), sizeof(int) * count, hipMemcpyDeviceToHost,
+                          st_));
+  }
   HIPCHK(hipStreamSynchronize(st_));
-  return h;
+  if (hint[0]) {
+    ch.clear_solve_error();
+    throw Error(HYKKT_E_STATE, "triangular-solve sweep exceeded its wait bound (results invalid)");
+  }
+  at = 1;
+  if (zrow) {
+    zrow->assign(hint + at, hint + at + count);
+    at += count;
+  }
+  if (piv) {
+    piv->assign(hint + at, hint + at + count);
+    for (int& v : *piv) v = (v == INT_MAX) ? -1 : v;
+  }
+  return std::vector<double>(hres, hres + nres);
 }
 
 void Solver::get_solution(int k, double* dx, double* ds, double* dyc, double* dyd)

@@ -91,13 +91,18 @@ class Solver {
   };
   std::vector<CgResult> solve_phase(const Pass& w, int64_t rxt, int64_t ysc, int64_t rs,
                                     int64_t ryd, bool x0, int64_t dx, int64_t dsv, int64_t dyc,
-                                    int64_t dyd, const dev::Batch& b, hipEvent_t after_cg);
+                                    int64_t dyd, const dev::Batch& b, hipEvent_t after_cg,
+                                    int iters_hint);
   // Residual rows of (dx, dsv, dyc, dyd) (arena offsets) for the systems of
   // b; rows: keep the signed rows for refinement.  Enqueues only; the eight
   // squared norms of each system are at its `res` slot.
   void residual(int64_t dx, int64_t dsv, int64_t dyc, int64_t dyd, bool rows, const dev::Batch& b);
-  // The res slots of `count` systems to the host (synchronizes).
-  std::vector<double> read_res(int count);
+  // The res slots of `count` systems to the host in one round trip
+  // (synchronizes; throws if a sweep exceeded its wait bound); with zrow /
+  // piv, also each system's zero-row flag (INT_MAX: none) and first failing
+  // pivot (-1: none).
+  std::vector<double> read_res(int count, std::vector<int>* zrow = nullptr,
+                               std::vector<int>* piv = nullptr);
   double* P(int64_t off) const;   // system 0's address of an arena slot
   // Run the launch in fn bracketed for the profiler, charged with the work
   // of the b.count systems it covers.
@@ -119,7 +124,10 @@ class Solver {
   int vec_n_[8] = {};
   std::vector<SystemIn> sys_in_;
   std::vector<hykkt_stats> st_k_;
-  int last_iters_ = 0;   // CG iterations of the last pass (first host check)
+  // CG iterations of the last first pass and of the last refinement pass:
+  // where the host first looks at the live flags in the next such pass
+  int last_iters_ = 0;
+  int last_ref_iters_ = 0;
   hykkt_tuning tuning_{};
   // file-path storage (read_matrix_files)
   CsrHost fblk_[4];

@@ -23,6 +23,7 @@ namespace {
 constexpr int NB = 128;   // panel step, MFMA tile edge, solve block
 constexpr int SR = 32;    // rows of a short tile
 constexpr int GMAX = 8;   // most steps one grouped trailing update can cover
+constexpr int BWD_OWN_CHUNKS = 2;   // see build_solve
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 }  // namespace
 
@@ -421,10 +422,16 @@ void CholEngine::build_solve()
   std::vector<int64_t> bpo(ns + 1, 0);
   for (int s = ns - 1; s >= 0; s--) {   // parents first: row chunks, then pivot blocks
     std::vector<int>& b = top[s] ? bwd_top : bwd;
-    const int np = cdiv(pl_.nc(s), NB);
-    for (int u = 0; u < cdiv(pl_.nr(s), NB); u++)
+    const int np = cdiv(pl_.nc(s), NB), nu = cdiv(pl_.nr(s), NB);
+    if (!top[s] && nu <= BWD_OWN_CHUNKS) {
+      // a forest supernode (one block) with few row chunks: its pivot item
+      // forms the chunks' products itself, one dependency link instead of two
+      b.insert(b.end(), {s, 2, 0});
+      continue;
+    }
+    for (int u = 0; u < nu; u++)
       for (int k = 0; k < np; k++) b.insert(b.end(), {s, 0, u * np + k});
-    for (int k = cdiv(pl_.nc(s), NB) - 1; k >= 0; k--) b.insert(b.end(), {s, 1, k});
+    for (int k = np - 1; k >= 0; k--) b.insert(b.end(), {s, 1, k});
   }
   for (int s = 0; s < ns; s++)
     bpo[s + 1] = bpo[s] + static_cast<int64_t>(cdiv(pl_.nr(s), NB)) * pl_.nc(s);

@@ -20,7 +20,11 @@
 //     (all chunks of all column blocks run in parallel);
 //   pivot item (s, b): sums the chunk partials in a fixed order, then
 //     L(j, b)^T x_j over the later pivot blocks j > b as they finish, and
-//     x_b = Dinv_b^T (t - acc).
+//     x_b = Dinv_b^T (t - acc);
+//   a one-block supernode of the small-supernode forest with at most two row
+//     chunks has no chunk items: its pivot item forms the chunks' products
+//     itself, in the same order and arithmetic, so the backward forest
+//     carries one dependency link per tree level instead of two.
 // The L block of the next step, and the inverted diagonal block before the
 // last wait, are loaded into registers before the item waits, so a link of
 // the dependency chain costs the flag round trip and arithmetic only.
@@ -684,7 +688,26 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   }
 
   double t = 0.0;
-  if (q == 0 && live) {
+  if (kind == 2) {
+    // a one-block supernode whose item forms its row chunks' products
+    // L(rows, :)^T x(rows) itself, chunk by chunk in the order (and with the
+    // arithmetic) of the chunk items it replaces
+    if (q == 0 && live) t = y[si.a + col];
+    for (int u = 0; u < si.nu; u++) {
+      const int rr0 = u * SB;
+      const int nrows = min(SB, si.nr - rr0);
+      const int* rows = c.rows + c.rows_ptr[s] + rr0;
+      __syncthreads();   // the previous chunk's reads of xs and part are done
+      if (threadIdx.x < SB)
+        xs[threadIdx.x] = (threadIdx.x < nrows) ? spin_load(x + rows[threadIdx.x], sd.err) : 0.0;
+      __syncthreads();
+      const uint32_t so = static_cast<uint32_t>(si.nc + rr0 + q * QW) * 8u;
+      part[q][col] = slice_dot_stream<QW>(lv, cvo, so, 8u, nrows - q * QW, live, xs + q * QW);
+      __syncthreads();
+      if (q == 0) t -= sum_slices<NQ>(part, col);
+    }
+    __syncthreads();   // every read of part and xs before they are written below
+  } else if (q == 0 && live) {
     t = y[si.a + c0 + col];
     const double* bp = B.bpart + sd.bpart_off[s] + c0 + col;
     for (int u = 0; u < si.nu; u++) t -= spin_load(bp + static_cast<int64_t>(u) * si.nc, sd.err);

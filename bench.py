@@ -153,7 +153,7 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0):
     from hykkt_amd.served import Server, make_patterns, poisson_requests, summarize
     t0 = time.perf_counter()
     pats = make_patterns(npat, seed=100)
-    srv = Server(pats, hbm_budget=budget)
+    srv = Server(pats, hbm_budget=budget, sweep_share=8)
     setup = time.perf_counter() - t0
     print(f"bench: served setup {setup:.1f} s, {len(srv.served)} patterns, "
           f"{srv.hbm_used / 1e9:.1f} GB, capacities {srv.capacity}", file=sys.stderr, flush=True)

@@ -42,7 +42,7 @@ class Options(C.Structure):
                 ("cg_tol", C.c_double), ("cg_itmax", C.c_int), ("delta", C.c_double),
                 ("inputs_on_device", C.c_int), ("cg_check_every", C.c_int),
                 ("verbose", C.c_int), ("refine_max", C.c_int), ("refine_tol", C.c_double),
-                ("autotune", C.c_int)]
+                ("autotune", C.c_int), ("sweep_share", C.c_int)]
 
 
 class Stats(C.Structure):

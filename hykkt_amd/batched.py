@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from hykkt_amd.served import Request, Server, ServedPattern, summarize
+from hykkt_amd.served import SWEEP_SHARE, Request, Server, ServedPattern, summarize
 
 # 256 systems: 32 patterns x 8 value draws, bus counts uniform on [200, 5000]
 # (branches 1.3x buses), gamma log-uniform on [1e4, 1e6] per system, Ds
@@ -52,10 +52,12 @@ def batch_requests(patterns):
     return reqs
 
 
-def make_server(patterns, hbm_budget: float = 200e9, keep_solutions: bool = False):
+def make_server(patterns, hbm_budget: float = 200e9, keep_solutions: bool = False,
+                sweep_share: int = SWEEP_SHARE):
     """One batch slot per pattern holding all its draws."""
     draws = max(len(p.pool) for p in patterns)
-    return Server(patterns, hbm_budget, capacity=draws, keep_solutions=keep_solutions)
+    return Server(patterns, hbm_budget, capacity=draws, keep_solutions=keep_solutions,
+                  sweep_share=sweep_share)
 
 
 def run_batch(server: Server, patterns, plain: bool = False):

@@ -77,6 +77,7 @@ int hykkt_default_options(hykkt_options* o)
     o->refine_max = 3;
     o->refine_tol = 9e-9;   // the 1e-8 accuracy target less a 10% margin
     o->autotune = 1;
+    o->sweep_share = 1;
     return HYKKT_OK;
   });
 }
@@ -204,6 +205,7 @@ int hykkt_set_options(hykkt_solver s, const hykkt_options* o)
     HYKKT_REQUIRE(o->ruiz_its >= 0 && o->cg_itmax >= 0 && o->cg_check_every >= 0 &&
                       o->refine_max >= 0,
                   HYKKT_E_ARG, "negative iteration count");
+    HYKKT_REQUIRE(o->sweep_share >= 1, HYKKT_E_ARG, "sweep_share must be at least 1");
     s->impl.opts = *o;
     return HYKKT_OK;
   });

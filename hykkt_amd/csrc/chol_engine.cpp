@@ -483,7 +483,9 @@ void CholEngine::build_solve()
   sd_.bpart_off = bpart_off_.p;
   sd_.err = flags_.p;
   sd_.ticket = flags_.p + 1;
-  sd_.grid_fwd = n_cu_ * dev::forest_blocks_per_cu();
+  grid_fwd_full_ = n_cu_ * dev::forest_blocks_per_cu();
+  sd_.grid_fwd = grid_fwd_full_;
+  share_ = 1;
   n_bpart_ = bpo[ns];
   dev::fill_i(flags_.p, 0, 1, st_);
 }
@@ -602,6 +604,23 @@ std::vector<int> CholEngine::pivots(int count)
 }
 
 void CholEngine::clear_solve_error() { dev::fill_i(sd_.err, 0, 1, st_); }
+
+void CholEngine::set_sweep_share(int share)
+{
+  share = std::max(share, 1);
+  if (share == share_) return;
+  share_ = share;
+  if (share == 1) {
+    sd_.grid_fwd = grid_fwd_full_;
+    sd_.cap_forest = sd_.cap_top = 0;
+    return;
+  }
+  // the forest sweeps: their share of the 256-thread workgroups the device
+  // holds; the top of the tree: of two 1024-thread workgroups per CU
+  sd_.grid_fwd = std::max(1, grid_fwd_full_ / share);
+  sd_.cap_forest = sd_.grid_fwd;
+  sd_.cap_top = std::max(1, 2 * n_cu_ / share);
+}
 
 int CholEngine::solve_error()
 {

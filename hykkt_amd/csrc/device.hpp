@@ -122,6 +122,10 @@ class CholEngine {
   // Rebuild the task lists with launch shapes t for launches of `count` systems.
   void set_tuning(const Tuning& t, int count);
   const Tuning& tuning() const { return tun_; }
+  // Let each sweep launch hold at most 1/share of the device's workgroup
+  // slots (share <= 1: no bound), leaving compute units to the kernels of
+  // other solvers' streams; the sweeps' results do not depend on it.
+  void set_sweep_share(int share);
   // Device words the solver reads back with its other per-pass results:
   // each system's first failing pivot (INT_MAX: none) and the sweeps' wait-
   // bound error word (0: none; clear_solve_error resets it).
@@ -183,6 +187,8 @@ class CholEngine {
   Tuning tun_;
   bool pinned_ = false;
   int tcount_ = 1;   // systems per launch the task lists are shaped for
+  int grid_fwd_full_ = 1;   // the forward forest sweep's workgroups at full share
+  int share_ = 1;
   DevBuf<double> own_;   // owned storage (own)
   DevBuf<int> own_piv_;
   hipStream_t st2_ = nullptr;   // look-ahead: the rest of each trailing update

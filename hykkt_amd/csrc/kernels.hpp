@@ -254,6 +254,8 @@ struct SolveDev {
   int* ticket;              // 4 item counters (forward forest, forward top, backward top,
                             // backward forest), zeroed with the first sentinel fill of a solve
   int grid_fwd;             // workgroups of the persistent forward forest sweep (resident at once)
+  int cap_top = 0;          // most workgroups of a top-of-tree sweep launch (0: one per item)
+  int cap_forest = 0;       // most workgroups of a backward forest sweep launch (0: one per item)
 };
 // Workgroups of the forward forest sweep one compute unit holds at once.
 int forest_blocks_per_cu();

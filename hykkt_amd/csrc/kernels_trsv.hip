@@ -174,7 +174,7 @@ __device__ __forceinline__ int take_item(int* ticket)
 
 // Fixed-order sum of the NQ slice partials of one row (column).
 template <int NQ>
-__device__ __forceinline__ double sum_slices(const double (&part)[NQ][SB], int i)
+__device__ __forceinline__ double sum_slices(const double (*part)[SB], int i)
 {
   double a = 0.0, b = 0.0;
 #pragma unroll
@@ -273,15 +273,13 @@ __device__ __forceinline__ double slice_dot_stream(rsrc_t rs, uint32_t vo, uint3
 // the tree, NT_TOP); otherwise slices are streamed after each wait.  The
 // small-supernode forest has its own persistent kernel (k_fwd_forest).
 template <int NT, bool PF>
-__global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
-                                                  const int* __restrict__ items,
-                                                  const double* __restrict__ rhs,
-                                                  double* __restrict__ yout, Batch bt)
+__device__ __forceinline__ void fwd_item(const CholDev& c, const SolveDev& sd,
+                                         const int* __restrict__ items,
+                                         const double* __restrict__ rhs,
+                                         double* __restrict__ yout, const Batch& bt, int itc,
+                                         double* yv, double (*part)[SB])
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
-  __shared__ double yv[SB];
-  __shared__ double part[NQ][SB];
-  const int itc = take_item(sd.ticket + (PF ? 1 : 0));
   const int sy = itc % bt.count, it = itc / bt.count;
   if (bt.mask && !bt.mask[sy]) return;
   const SysBufs B = sys_bufs(c, sd, sy, bt, rhs, yout);
@@ -361,6 +359,27 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
     if (q == 0 && live) xstore(y + si.a + r, sum_slices<NQ>(part, row));
   } else {
     if (q == 0 && live) xstore(B.ubuf + sd.uvec_off[s] + (r - si.nc), v);
+  }
+}
+
+// Persistent: a workgroup takes items from the sweep's counter until they
+// run out, so the grid may be smaller than the item list (fewer resident
+// workgroups leave compute units to other streams); the order argument above
+// holds for any grid size.
+template <int NT, bool PF>
+__global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
+                                                  const int* __restrict__ items,
+                                                  const double* __restrict__ rhs,
+                                                  double* __restrict__ yout, Batch bt, int total)
+{
+  constexpr int NQ = NT / SB;
+  __shared__ double yv[SB];
+  __shared__ double part[NQ][SB];
+  for (;;) {
+    const int itc = take_item(sd.ticket + (PF ? 1 : 0));
+    if (itc >= total) break;
+    fwd_item<NT, PF>(c, sd, items, rhs, yout, bt, itc, yv, part);
+    __syncthreads();   // the item's last reads of yv and part
   }
 }
 
@@ -608,15 +627,13 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
 }
 
 template <int NT, bool PF>
-__global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
-                                                  const int* __restrict__ items,
-                                                  const double* __restrict__ yin,
-                                                  double* __restrict__ xout, Batch bt)
+__device__ __forceinline__ void bwd_item(const CholDev& c, const SolveDev& sd,
+                                         const int* __restrict__ items,
+                                         const double* __restrict__ yin,
+                                         double* __restrict__ xout, const Batch& bt, int itc,
+                                         double* xs, double (*part)[SB], double* wl)
 {
   constexpr int NQ = NT / SB, QW = SB / NQ;
-  __shared__ double xs[SB];
-  __shared__ double part[NQ][SB];
-  const int itc = take_item(sd.ticket + (PF ? 2 : 3));
   const int sy = itc % bt.count, it = itc / bt.count;
   if (bt.mask && !bt.mask[sy]) return;
   const SysBufs B = sys_bufs(c, sd, sy, bt, yin, xout);
@@ -624,7 +641,6 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   const int kind = uni(items[3 * it + 1]);
   const int idx = uni(items[3 * it + 2]);
   if constexpr (PF) {
-    extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS)
     solve_bwd_top(c, sd, B, s, kind, idx, wl);
     return;
   }
@@ -738,6 +754,24 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
 }
 
+template <int NT, bool PF>
+__global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
+                                                  const int* __restrict__ items,
+                                                  const double* __restrict__ yin,
+                                                  double* __restrict__ xout, Batch bt, int total)
+{
+  constexpr int NQ = NT / SB;
+  __shared__ double xs[SB];
+  __shared__ double part[NQ][SB];
+  extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS; top only)
+  for (;;) {
+    const int itc = take_item(sd.ticket + (PF ? 2 : 3));
+    if (itc >= total) break;
+    bwd_item<NT, PF>(c, sd, items, yin, xout, bt, itc, xs, part, wl);
+    __syncthreads();   // the item's last reads of the shared arrays
+  }
+}
+
 }  // namespace
 
 void fill_sentinel(double* p, int64_t n, const Batch& b, hipStream_t st, int* zero4)
@@ -761,6 +795,14 @@ void fill_sentinel3(double* p0, int64_t n0, double* p1, int64_t n1, double* p2, 
   HIPCHK(hipGetLastError());
 }
 
+// Workgroups of a persistent sweep launch: one per item, at most `cap` (0:
+// no bound).
+static unsigned sweep_grid(int items, int cap)
+{
+  const int g = cap > 0 ? std::min(items, cap) : items;
+  return static_cast<unsigned>(std::max(g, 1));
+}
+
 int forest_blocks_per_cu()
 {
   int nb = 0;
@@ -776,8 +818,8 @@ void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double
     k_fwd_forest<<<std::min(sd.n_fwd * bt.count, sd.grid_fwd), NT_FOREST, 0, st>>>(
         c, sd, sd.fwd_items, b, y, bt);
   if (part == 1 && sd.n_fwd_top > 0)
-    k_solve_fwd<NT_TOP, true><<<sd.n_fwd_top * bt.count, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top,
-                                                                          b, y, bt);
+    k_solve_fwd<NT_TOP, true><<<sweep_grid(sd.n_fwd_top * bt.count, sd.cap_top), NT_TOP, 0,
+                                st>>>(c, sd, sd.fwd_items_top, b, y, bt, sd.n_fwd_top * bt.count);
   HIPCHK(hipGetLastError());
 }
 
@@ -794,12 +836,12 @@ void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, doubl
       return true;
     }();
     (void)attr;
-    k_solve_bwd<NT_TOP, true><<<sd.n_bwd_top * bt.count, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top,
-                                                                           y, x, bt);
+    k_solve_bwd<NT_TOP, true><<<sweep_grid(sd.n_bwd_top * bt.count, sd.cap_top), NT_TOP, WL,
+                                st>>>(c, sd, sd.bwd_items_top, y, x, bt, sd.n_bwd_top * bt.count);
   }
   if (part == 0 && sd.n_bwd > 0)
-    k_solve_bwd<NT_FOREST, false><<<sd.n_bwd * bt.count, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items,
-                                                                             y, x, bt);
+    k_solve_bwd<NT_FOREST, false><<<sweep_grid(sd.n_bwd * bt.count, sd.cap_forest), NT_FOREST,
+                                    0, st>>>(c, sd, sd.bwd_items, y, x, bt, sd.n_bwd * bt.count);
   HIPCHK(hipGetLastError());
 }
 

@@ -717,6 +717,7 @@ std::vector<int> Solver::run(int count, bool from_setters)
 
   // ---- H_gamma straight into the factor storage (HykktSolver.cpp:370-395) ----
   CholEngine& ch = *S.chol;
+  ch.set_sweep_share(opts.sweep_share);
   assemble_hgamma(b);
 
   // ---- compute_hgamma_factorization (HykktSolver.cpp:398-401) ----

@@ -47,6 +47,10 @@ from hykkt_amd.generator import GridPattern, ds_spread_schedule
 
 # Bus counts of the served mix and their shares (6:3:1).
 SERVED_SIZES = ((1_000, 0.6), (5_000, 0.3), (20_000, 0.1))
+# The batch slots' sweeps each hold at most 1/SWEEP_SHARE of the device's
+# workgroup slots, so the slots' latency-bound sweeps overlap; the plain
+# solvers, one solve at a time, keep the whole device.
+SWEEP_SHARE = 4
 # Value sets per served pattern: request i of a pattern uses entry i mod POOL,
 # whose Ds spread grows with the entry (the interior-point step index).
 POOL = 20
@@ -161,7 +165,7 @@ class Server:
     slot (a function of the bus count, default_capacity by default)."""
 
     def __init__(self, patterns, hbm_budget: float, capacity=None, queue_batches: int = 4,
-                 keep_solutions: bool = False):
+                 keep_solutions: bool = False, sweep_share: int = SWEEP_SHARE):
         self.patterns = {p.pid: p for p in patterns}
         self.keep = keep_solutions
         self.queue_batches = queue_batches
@@ -177,7 +181,7 @@ class Server:
             cap = max(1, min(cap_of(p.nbus), len(p.pool)))
             free0, _ = H.device_mem_info()
             b = H.BatchSolver(p.g.gamma, cap)
-            b.options(inputs_on_device=1)
+            b.options(inputs_on_device=1, sweep_share=sweep_share)
             s = H.HykktSolver(p.g.gamma)
             s.options(inputs_on_device=1)
             p.set_pattern(b, 0)

@@ -123,6 +123,10 @@ typedef struct {
                                The timing runs are left out of the stats' times and of the
                                kernel profile.  HYKKT_PAIR_MIN / HYKKT_GROUP_MAX in the
                                environment fix the shapes instead */
+  int    sweep_share;       /* 1 (default): the triangular-solve sweeps may hold every
+                               workgroup slot of the device; k > 1: each sweep launch holds at
+                               most 1/k of them, leaving compute units to other solvers'
+                               kernels (many solvers sharing the device); same results */
 } hykkt_options;
 
 typedef struct {

@@ -187,3 +187,17 @@ def test_batch_refine_and_cg_options():
     for opts in (dict(refine_max=0), dict(cg_check_every=2), dict(cg_check_every=1)):
         b, rc = batched(g, systems, **opts)
         check_same(g, systems, b, rc, **opts)
+
+
+def test_sweep_share_same_bits():
+    """Sweeps bounded to a share of the device's workgroup slots (persistent
+    workgroups taking the items in order) give the same bits as unbounded
+    ones, batched and alone."""
+    g = GridPattern(400, 520, seed=21, gamma=1e5)
+    systems = draws(g, 3, [(1e-4, 1e8), (1e-2, 1e2)], [1e5, 3e4])
+    b1, rc1 = batched(g, systems)
+    b8, rc8 = batched(g, systems, sweep_share=8)
+    assert rc1 == rc8
+    for k in range(len(systems)):
+        assert np.array_equal(np.concatenate(b1.solution_k(k)), np.concatenate(b8.solution_k(k)))
+    check_same(g, systems, b8, rc8, sweep_share=8)

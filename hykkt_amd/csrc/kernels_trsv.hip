@@ -362,11 +362,12 @@ __device__ __forceinline__ void fwd_item(const CholDev& c, const SolveDev& sd,
   }
 }
 
-// Persistent: a workgroup takes items from the sweep's counter until they
-// run out, so the grid may be smaller than the item list (fewer resident
+// LOOP: persistent; a workgroup takes items from the sweep's counter until
+// they run out, so the grid may be smaller than the item list (fewer resident
 // workgroups leave compute units to other streams); the order argument above
-// holds for any grid size.
-template <int NT, bool PF>
+// holds for any grid size.  Otherwise one item per workgroup, one workgroup
+// per item (fewer registers on the item's path).
+template <int NT, bool PF, bool LOOP>
 __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
                                                   const double* __restrict__ rhs,
@@ -375,6 +376,10 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
   constexpr int NQ = NT / SB;
   __shared__ double yv[SB];
   __shared__ double part[NQ][SB];
+  if constexpr (!LOOP) {
+    fwd_item<NT, PF>(c, sd, items, rhs, yout, bt, take_item(sd.ticket + (PF ? 1 : 0)), yv, part);
+    return;
+  }
   for (;;) {
     const int itc = take_item(sd.ticket + (PF ? 1 : 0));
     if (itc >= total) break;
@@ -754,7 +759,7 @@ __device__ __forceinline__ void bwd_item(const CholDev& c, const SolveDev& sd,
   if (q == 0 && live) xstore(x + si.a + c0 + col, sum_slices<NQ>(part, col));
 }
 
-template <int NT, bool PF>
+template <int NT, bool PF, bool LOOP>
 __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
                                                   const int* __restrict__ items,
                                                   const double* __restrict__ yin,
@@ -764,6 +769,11 @@ __global__ void __launch_bounds__(NT) k_solve_bwd(CholDev c, SolveDev sd,
   __shared__ double xs[SB];
   __shared__ double part[NQ][SB];
   extern __shared__ double wl[];   // 128 x 128 inverse block (dynamic LDS; top only)
+  if constexpr (!LOOP) {
+    bwd_item<NT, PF>(c, sd, items, yin, xout, bt, take_item(sd.ticket + (PF ? 2 : 3)), xs, part,
+                     wl);
+    return;
+  }
   for (;;) {
     const int itc = take_item(sd.ticket + (PF ? 2 : 3));
     if (itc >= total) break;
@@ -817,10 +827,30 @@ void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double
   if (part == 0 && sd.n_fwd > 0)
     k_fwd_forest<<<std::min(sd.n_fwd * bt.count, sd.grid_fwd), NT_FOREST, 0, st>>>(
         c, sd, sd.fwd_items, b, y, bt);
-  if (part == 1 && sd.n_fwd_top > 0)
-    k_solve_fwd<NT_TOP, true><<<sweep_grid(sd.n_fwd_top * bt.count, sd.cap_top), NT_TOP, 0,
-                                st>>>(c, sd, sd.fwd_items_top, b, y, bt, sd.n_fwd_top * bt.count);
+  if (part == 1 && sd.n_fwd_top > 0) {
+    const int total = sd.n_fwd_top * bt.count;
+    if (sd.cap_top > 0)
+      k_solve_fwd<NT_TOP, true, true><<<sweep_grid(total, sd.cap_top), NT_TOP, 0, st>>>(
+          c, sd, sd.fwd_items_top, b, y, bt, total);
+    else
+      k_solve_fwd<NT_TOP, true, false><<<total, NT_TOP, 0, st>>>(c, sd, sd.fwd_items_top, b, y,
+                                                                 bt, total);
+  }
   HIPCHK(hipGetLastError());
+}
+
+// Raise the top backward sweep's dynamic LDS limit, once per process and
+// kernel variant (a function-local static initialises thread-safely).
+template <bool LOOP>
+static void allow_bwd_top_lds(size_t bytes)
+{
+  static const bool done = [bytes] {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true, LOOP>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(bytes)));
+    return true;
+  }();
+  (void)done;
 }
 
 void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
@@ -829,19 +859,26 @@ void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, doubl
   if (bt.count <= 0) return;
   if (part == 1 && sd.n_bwd_top > 0) {
     constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);   // staged inverse block
-    // set once per process; a function-local static initialises thread-safely
-    static const bool attr = [] {
-        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(WL)));
-      return true;
-    }();
-    (void)attr;
-    k_solve_bwd<NT_TOP, true><<<sweep_grid(sd.n_bwd_top * bt.count, sd.cap_top), NT_TOP, WL,
-                                st>>>(c, sd, sd.bwd_items_top, y, x, bt, sd.n_bwd_top * bt.count);
+    const int total = sd.n_bwd_top * bt.count;
+    if (sd.cap_top > 0) {
+      allow_bwd_top_lds<true>(WL);
+      k_solve_bwd<NT_TOP, true, true><<<sweep_grid(total, sd.cap_top), NT_TOP, WL, st>>>(
+          c, sd, sd.bwd_items_top, y, x, bt, total);
+    } else {
+      allow_bwd_top_lds<false>(WL);
+      k_solve_bwd<NT_TOP, true, false><<<total, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top, y, x,
+                                                                  bt, total);
+    }
   }
-  if (part == 0 && sd.n_bwd > 0)
-    k_solve_bwd<NT_FOREST, false><<<sweep_grid(sd.n_bwd * bt.count, sd.cap_forest), NT_FOREST,
-                                    0, st>>>(c, sd, sd.bwd_items, y, x, bt, sd.n_bwd * bt.count);
+  if (part == 0 && sd.n_bwd > 0) {
+    const int total = sd.n_bwd * bt.count;
+    if (sd.cap_forest > 0)
+      k_solve_bwd<NT_FOREST, false, true><<<sweep_grid(total, sd.cap_forest), NT_FOREST, 0,
+                                            st>>>(c, sd, sd.bwd_items, y, x, bt, total);
+    else
+      k_solve_bwd<NT_FOREST, false, false><<<total, NT_FOREST, 0, st>>>(c, sd, sd.bwd_items, y, x,
+                                                                        bt, total);
+  }
   HIPCHK(hipGetLastError());
 }
 

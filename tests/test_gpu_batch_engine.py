@@ -201,3 +201,6 @@ def test_sweep_share_same_bits():
     for k in range(len(systems)):
         assert np.array_equal(np.concatenate(b1.solution_k(k)), np.concatenate(b8.solution_k(k)))
     check_same(g, systems, b8, rc8, sweep_share=8)
+    with pytest.raises(H.HykktError) as ex:
+        b8.options(sweep_share=0)
+    assert ex.value.code == -1

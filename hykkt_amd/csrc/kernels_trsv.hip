@@ -329,24 +329,27 @@ __device__ __forceinline__ void fwd_item(const CholDev& c, const SolveDev& sd,
     for (int p = sd.pull_ptr[fr]; p < sd.pull_ptr[fr + 1]; p++)
       v += spin_load(B.ubuf + sd.pull_src[p], sd.err);
   }
-  // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready
+  // v -= L(r, B_j) y_j over the earlier pivot blocks, as each y_j is ready.
+  // Step j's slice products go to part[j & 1]: it is written again two steps
+  // on, after the next step's barriers, so a step needs two barriers.
   for (int j = 0; j < jmax; j++) {
     const int bj = min(SB, si.nc - j * SB);
+    double (*pj)[SB] = part + (j & 1) * NQ;
     if (threadIdx.x < SB)
       yv[threadIdx.x] = (threadIdx.x < bj) ? spin_load(y + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
     __syncthreads();
     double tp;
     if constexpr (PF) tp = slice_dot<QW>(l, yv + q * QW);
     else tp = slice_dot_stream<QW>(lview(j), vo, so, f8, lvalid(j), live, yv + q * QW);
-    part[q][row] = tp;
+    pj[q][row] = tp;
     __syncthreads();
     // the barrier keeps the next block's loads after the last use of l
     if constexpr (PF) {
       if (j + 1 < jmax) slice_load<QW>(l, lview(j + 1), vo, so, f8, lvalid(j + 1), live);
     }
-    if (q == 0) v -= sum_slices<NQ>(part, row);
-    __syncthreads();
+    if (q == 0) v -= sum_slices<NQ>(pj, row);
   }
+  __syncthreads();   // part and yv are written again below
   if (kind == 0) {
     // y = Dinv_blk v
     if (q == 0) yv[row] = live ? v : 0.0;
@@ -375,7 +378,7 @@ __global__ void __launch_bounds__(NT) k_solve_fwd(CholDev c, SolveDev sd,
 {
   constexpr int NQ = NT / SB;
   __shared__ double yv[SB];
-  __shared__ double part[NQ][SB];
+  __shared__ double part[2 * NQ][SB];
   if constexpr (!LOOP) {
     fwd_item<NT, PF>(c, sd, items, rhs, yout, bt, take_item(sd.ticket + (PF ? 1 : 0)), yv, part);
     return;

@@ -492,10 +492,65 @@ __global__ void __launch_bounds__(NT_FOREST) k_fwd_forest(CholDev c, SolveDev sd
 // (row r = tid % 128, column slice q = tid / 128) holds acc[k], its row's
 // term for column 16 q + k.  Within each wave the 16 values are
 // transpose-reduced over the 64 lanes (each exchange halves the values a
-// lane keeps), then the two row halves of a slice meet in LDS.  Returns with
-// out[c] = sum over the 128 rows for column c (all threads past a barrier).
-// One exchange step: the N values of a lane become N/2 (lanes with bit M set
-// keep the upper half, the others the lower half; each adds its partner's).
+// lane keeps: the lanes with bit M of the lane index set keep the upper
+// half, the others the lower half, and each adds its partner's, lane
+// l ^ M), then the two row halves of a slice meet in LDS.  The exchanges run
+// on the VALU: the lane-half swaps of gfx950 (v_permlane32_swap,
+// v_permlane16_swap) for M = 32 and 16, DPP row rotations and shifts for
+// M = 8 and 4, DPP quad permutations for the last two full sums.
+__device__ __forceinline__ uint32_t lo32(double v)
+{
+  return static_cast<uint32_t>(__builtin_bit_cast(uint64_t, v));
+}
+__device__ __forceinline__ uint32_t hi32(double v)
+{
+  return static_cast<uint32_t>(__builtin_bit_cast(uint64_t, v) >> 32);
+}
+__device__ __forceinline__ double mk64(uint32_t lo, uint32_t hi)
+{
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// One exchange for M = 32 (W32) or 16 over the pairs (acc[i], acc[i + N/2]):
+// the swap puts a lane's kept value and its partner's sent value in the
+// two registers, whose sum is keep + partner's value (addition commutes).
+template <int N, bool W32>
+__device__ __forceinline__ void halve_swap(double (&acc)[16])
+{
+#pragma unroll
+  for (int i = 0; i < N / 2; i++) {
+    const double x = acc[i], y = acc[i + N / 2];
+    const auto lo = W32 ? __builtin_amdgcn_permlane32_swap(lo32(x), lo32(y), false, false)
+                        : __builtin_amdgcn_permlane16_swap(lo32(x), lo32(y), false, false);
+    const auto hi = W32 ? __builtin_amdgcn_permlane32_swap(hi32(x), hi32(y), false, false)
+                        : __builtin_amdgcn_permlane16_swap(hi32(x), hi32(y), false, false);
+    acc[i] = mk64(lo[0], hi[0]) + mk64(lo[1], hi[1]);
+  }
+}
+
+// The value of lane l ^ M (M = 8, 4, 2 or 1) by DPP: row_ror:8 for 8;
+// row_shr:4 / row_shl:4 by the lane's bit 2 for 4; quad_perm for 2 and 1.
+template <int M>
+__device__ __forceinline__ double xor_dpp(double v, int lane)
+{
+  const uint32_t a = lo32(v), b = hi32(v);
+  if constexpr (M == 8) {
+    return mk64(__builtin_amdgcn_update_dpp(0u, a, 0x128, 0xf, 0xf, false),
+                __builtin_amdgcn_update_dpp(0u, b, 0x128, 0xf, 0xf, false));
+  } else if constexpr (M == 4) {
+    const uint32_t dl = __builtin_amdgcn_update_dpp(0u, a, 0x114, 0xf, 0xf, false);
+    const uint32_t dh = __builtin_amdgcn_update_dpp(0u, b, 0x114, 0xf, 0xf, false);
+    const uint32_t ul = __builtin_amdgcn_update_dpp(0u, a, 0x104, 0xf, 0xf, false);
+    const uint32_t uh = __builtin_amdgcn_update_dpp(0u, b, 0x104, 0xf, 0xf, false);
+    return (lane & 4) ? mk64(dl, dh) : mk64(ul, uh);
+  } else {
+    constexpr int QP = M == 2 ? 0x4E : 0xB1;   // quad_perm [2,3,0,1] / [1,0,3,2]
+    return mk64(__builtin_amdgcn_update_dpp(0u, a, QP, 0xf, 0xf, false),
+                __builtin_amdgcn_update_dpp(0u, b, QP, 0xf, 0xf, false));
+  }
+}
+
+// One exchange for M = 8 or 4 over the pairs (acc[i], acc[i + N/2]).
 template <int N, int M>
 __device__ __forceinline__ void halve(double (&acc)[16], int lane)
 {
@@ -504,19 +559,21 @@ __device__ __forceinline__ void halve(double (&acc)[16], int lane)
   for (int i = 0; i < N / 2; i++) {
     const double send = hi ? acc[i] : acc[i + N / 2];
     const double keep = hi ? acc[i + N / 2] : acc[i];
-    acc[i] = keep + __shfl_xor(send, M);
+    acc[i] = keep + xor_dpp<M>(send, lane);
   }
 }
 
-__device__ __forceinline__ void column_sums(double (&acc)[16], double (*half)[SB], double* out)
+// The two halves of the column sums: half[h][c] for the rows of wave
+// parity h; all threads past a barrier.
+__device__ __forceinline__ void column_halves(double (&acc)[16], double (*half)[SB])
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  halve<16, 32>(acc, lane);
-  halve<8, 16>(acc, lane);
+  halve_swap<16, true>(acc);
+  halve_swap<8, false>(acc);
   halve<4, 8>(acc, lane);
   halve<2, 4>(acc, lane);
-  acc[0] += __shfl_xor(acc[0], 2);
-  acc[0] += __shfl_xor(acc[0], 1);
+  acc[0] += xor_dpp<2>(acc[0], lane);
+  acc[0] += xor_dpp<1>(acc[0], lane);
   // lane l now holds column k(l) = 8 b5 + 4 b4 + 2 b3 + b2 of its slice
   if ((lane & 3) == 0) {
     const int k = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
@@ -524,6 +581,12 @@ __device__ __forceinline__ void column_sums(double (&acc)[16], double (*half)[SB
     half[wave & 1][(wave >> 1) * 16 + k] = acc[0];
   }
   __syncthreads();
+}
+
+// out[c] = the sum over the 128 rows for column c (all threads past a barrier).
+__device__ __forceinline__ void column_sums(double (&acc)[16], double (*half)[SB], double* out)
+{
+  column_halves(acc, half);
   if (threadIdx.x < SB) out[threadIdx.x] = half[0][threadIdx.x] + half[1][threadIdx.x];
   __syncthreads();
 }
@@ -543,6 +606,7 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
   __shared__ double xs[2][SB];
   __shared__ double half[2][SB];
   __shared__ double cs[SB];
+  __shared__ double ts[SB];
   __shared__ double part[8][SB];
   const double* __restrict__ y = B.in;
   double* __restrict__ x = B.out;
@@ -601,33 +665,38 @@ __device__ __forceinline__ void solve_bwd_top(const CholDev& c, const SolveDev& 
 #pragma unroll
     for (int k = 0; k < QW; k++) wl[(q * QW + k) * SB + r] = wt[k];
   }
+  auto rows_of = [&](int j) { return min(SB, si.nc - j * SB); };
   double l[QW];
-  if (jtop > blk) row_slice(l, jtop * SB, min(SB, si.nc - jtop * SB), c0, nb);
+  if (jtop > blk) row_slice(l, jtop * SB, rows_of(jtop), c0, nb);
 
-  double t = 0.0;
-  if (threadIdx.x < SB && threadIdx.x < nb) {
-    t = y[si.a + c0 + threadIdx.x];
-    const double* bp = B.bpart + sd.bpart_off[s] + c0 + threadIdx.x;
-    for (int uu = 0; uu < si.nu; uu++) t -= spin_load(bp + static_cast<int64_t>(uu) * si.nc, sd.err);
+  if (threadIdx.x < SB) {
+    double t = 0.0;
+    if (threadIdx.x < nb) {
+      t = y[si.a + c0 + threadIdx.x];
+      const double* bp = B.bpart + sd.bpart_off[s] + c0 + threadIdx.x;
+      for (int uu = 0; uu < si.nu; uu++) t -= spin_load(bp + static_cast<int64_t>(uu) * si.nc, sd.err);
+    }
+    ts[threadIdx.x] = t;
   }
   for (int j = jtop; j > blk; j--) {
-    const int bj = min(SB, si.nc - j * SB);
     double* xb = xs[j & 1];
     if (threadIdx.x < SB)
-      xb[threadIdx.x] = (threadIdx.x < bj) ? spin_load(x + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
+      xb[threadIdx.x] = (threadIdx.x < rows_of(j)) ? spin_load(x + si.a + j * SB + threadIdx.x, sd.err) : 0.0;
     __syncthreads();   // xs[j & 1] is written again two blocks on, after the next barrier
     const double xr = xb[r];
 #pragma unroll
     for (int k = 0; k < QW; k++) acc[k] += l[k] * xr;
-    if (j - 1 > blk) row_slice(l, (j - 1) * SB, min(SB, si.nc - (j - 1) * SB), c0, nb);
+    if (j - 1 > blk) row_slice(l, (j - 1) * SB, rows_of(j - 1), c0, nb);
   }
-  column_sums(acc, half, cs);
-  if (threadIdx.x < SB) xs[0][threadIdx.x] = (threadIdx.x < nb) ? t - cs[threadIdx.x] : 0.0;
-  __syncthreads();
+  // x = Dinv^T (t - column sums), the subtraction formed where it is used
+  column_halves(acc, half);
   {
     double res = 0.0;
 #pragma unroll
-    for (int k = 0; k < QW; k++) res += wl[(q * QW + k) * SB + r] * xs[0][q * QW + k];
+    for (int k = 0; k < QW; k++) {
+      const int cc = q * QW + k;
+      res += wl[cc * SB + r] * (ts[cc] - (half[0][cc] + half[1][cc]));
+    }
     part[q][r] = res;
   }
   __syncthreads();
@@ -808,6 +877,22 @@ void fill_sentinel3(double* p0, int64_t n0, double* p1, int64_t n1, double* p2, 
   HIPCHK(hipGetLastError());
 }
 
+// The top backward sweep stages one 128 x 128 inverse block in dynamic LDS.
+constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);
+
+// Raise kernel K's dynamic LDS limit to WL, once per process and kernel (a
+// function-local static initialises thread-safely).
+template <auto K>
+static void allow_top_lds()
+{
+  static const bool done = [] {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(K),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(WL)));
+    return true;
+  }();
+  (void)done;
+}
+
 // Workgroups of a persistent sweep launch: one per item, at most `cap` (0:
 // no bound).
 static unsigned sweep_grid(int items, int cap)
@@ -842,33 +927,19 @@ void solve_forward(const CholDev& c, const SolveDev& sd, const double* b, double
   HIPCHK(hipGetLastError());
 }
 
-// Raise the top backward sweep's dynamic LDS limit, once per process and
-// kernel variant (a function-local static initialises thread-safely).
-template <bool LOOP>
-static void allow_bwd_top_lds(size_t bytes)
-{
-  static const bool done = [bytes] {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_solve_bwd<NT_TOP, true, LOOP>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(bytes)));
-    return true;
-  }();
-  (void)done;
-}
 
 void solve_backward(const CholDev& c, const SolveDev& sd, const double* y, double* x, int part,
                     const Batch& bt, hipStream_t st)
 {
   if (bt.count <= 0) return;
   if (part == 1 && sd.n_bwd_top > 0) {
-    constexpr size_t WL = static_cast<size_t>(SB) * SB * sizeof(double);   // staged inverse block
     const int total = sd.n_bwd_top * bt.count;
     if (sd.cap_top > 0) {
-      allow_bwd_top_lds<true>(WL);
+      allow_top_lds<k_solve_bwd<NT_TOP, true, true>>();
       k_solve_bwd<NT_TOP, true, true><<<sweep_grid(total, sd.cap_top), NT_TOP, WL, st>>>(
           c, sd, sd.bwd_items_top, y, x, bt, total);
     } else {
-      allow_bwd_top_lds<false>(WL);
+      allow_top_lds<k_solve_bwd<NT_TOP, true, false>>();
       k_solve_bwd<NT_TOP, true, false><<<total, NT_TOP, WL, st>>>(c, sd, sd.bwd_items_top, y, x,
                                                                   bt, total);
     }

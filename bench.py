@@ -39,7 +39,7 @@ def metric_name():
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # Kernels whose bound is the FP64 pipes; the rest are priced against HBM.
-COMPUTE_BOUND = {"k_dense_nt", "k_potrf_inv"}
+COMPUTE_BOUND = {"k_dense_nt<128>", "k_dense_nt<32>", "k_potrf_inv"}
 
 
 def traffic_per_launch(kernel):

@@ -553,7 +553,7 @@ void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
         dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb, b,
                        st_);
       });
-      timed(PK_DENSE_NT, L.trsm[j], [&] {
+      timed(L.trsm[j].short_rows ? PK_DENSE_NT_SHORT : PK_DENSE_NT, L.trsm[j], [&] {
         dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
                       t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, b, st_,
                       L.trsm[j].short_rows);
@@ -563,7 +563,7 @@ void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
         rest_pending = false;
         rest_lo = INT_MAX;
       }
-      timed(PK_DENSE_NT, L.upd_next[j], [&] {
+      timed(L.upd_next[j].short_rows ? PK_DENSE_NT_SHORT : PK_DENSE_NT, L.upd_next[j], [&] {
         dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
                       t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, umode,
                       ustep, b, st_, L.upd_next[j].short_rows);

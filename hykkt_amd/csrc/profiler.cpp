@@ -15,7 +15,7 @@ const char* Profiler::name(int k)
 {
   // kernel names as rocprofv3 prints them (template instances of the sweeps)
   static const char* const names[PK_COUNT] = {
-      "k_dense_nt",           "k_potrf_inv",         "k_extend_add",
+      "k_dense_nt<128>",      "k_dense_nt<32>",      "k_potrf_inv",         "k_extend_add",
       "k_fwd_forest",         "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
       "k_solve_bwd<256, false>", "k_spmv",             "k_spgemm_values",
       "k_spgemm_scatter"};

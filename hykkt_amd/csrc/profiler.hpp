@@ -13,7 +13,8 @@
 namespace hykkt {
 
 enum ProfKind {
-  PK_DENSE_NT = 0,
+  PK_DENSE_NT = 0,       // 128 x 128 tiles
+  PK_DENSE_NT_SHORT,     // 32 x 128 tiles (the short launches on the critical path)
   PK_POTRF,
   PK_EXTEND_ADD,
   PK_SOLVE_FWD_FOREST,

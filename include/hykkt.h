@@ -264,8 +264,10 @@ int hykkt_device_mem_info(unsigned long long* free_bytes, unsigned long long* to
  * std::chrono in its examples).  While enabled, every launch of the kernels
  * below is bracketed by HIP events on the solver's stream and its duration
  * is summed with the algorithmic flops and HBM bytes it performs:
- *   k_dense_nt    MFMA tiles: TRSM (r*b^2), panel update and U = -L21 L21^T
- *                 (2*K per lower-triangle entry)
+ *   k_dense_nt<128> / k_dense_nt<32>  MFMA tiles of 128 or 32 rows x 128
+ *                 columns: TRSM (r*b^2), panel update and U = -L21 L21^T
+ *                 (2*K per lower-triangle entry); the 32-row tiles for the
+ *                 short launches on the factorization's critical path
  *   k_potrf_inv   128-column diagonal blocks (b^3/3)
  *   k_extend_add  update-matrix assembly (1 flop, 24 bytes per entry)
  *   k_solve_fwd / k_solve_bwd  one sweep each (2*nnz(L) flops, 8*nnz(L) bytes)

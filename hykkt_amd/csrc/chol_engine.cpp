@@ -4,8 +4,10 @@
 //   extend-add of the children's update matrices into the panels,
 //   blocked right-looking panel factorization in 128-column steps
 //     (factor + invert the diagonal block; TRSM of the rows below with the
-//      inverted block on the matrix cores; MFMA update of the panel's trailing
-//      columns),
+//      inverted block on the matrix cores -- for 128-wide blocks by blocked
+//      substitution with the inverses of the 16x16 diagonal blocks, the
+//      whole inverse formed after the level for the solves; MFMA update of
+//      the panel's trailing columns),
 //   U = -L21 L21^T on the matrix cores,
 //   extend-add of the children's update matrices into U.
 // Solves: one forward and one backward sweep over the whole tree.
@@ -174,7 +176,7 @@ void CholEngine::build_tasks()
   lv_.assign(nlev, Level{});
   std::vector<int> eap, eacol, eacptr{0}, eacc, eaccol;
   std::vector<int> potrf, trsm_sn, trsm_tr, trsm_tc, upd_sn, upd_tr, upd_tc;
-  std::vector<int> syrk_sn, syrk_tr, syrk_tc;
+  std::vector<int> syrk_sn, syrk_tr, syrk_tc, inv_sn, inv_step;
   struct Contrib {
     int t, c, r2;
   };
@@ -373,6 +375,22 @@ void CholEngine::build_tasks()
         }
     }
     L.syrk.cnt = static_cast<int>(syrk_sn.size()) - L.syrk.off;
+    // the whole inverses of the blocks of the steps factored without them
+    // (128-wide launches): b^3/3 flops, the triangle read twice and written
+    L.inv.off = static_cast<int>(inv_sn.size());
+    for (int j = 0; j < nsteps; j++) {
+      if (L.potrf[j].maxb <= 64) continue;
+      for (int q = b0; q < b1; q++) {
+        const int s = pl_.level_sn[q], nc = pl_.nc(s);
+        if (j * NB >= nc) continue;
+        const double db = std::min(NB, nc - j * NB);
+        inv_sn.push_back(s);
+        inv_step.push_back(j);
+        L.inv.flops += db * db * db / 3;
+        L.inv.bytes += 24.0 * db * (db + 1) / 2;
+      }
+    }
+    L.inv.cnt = static_cast<int>(inv_sn.size()) - L.inv.off;
   }
   auto up = [](DevBuf<int>& b, std::vector<int>& v) {
     if (v.empty()) v.push_back(0);
@@ -393,6 +411,8 @@ void CholEngine::build_tasks()
   up(t_syrk_sn_, syrk_sn);
   up(t_syrk_tr_, syrk_tr);
   up(t_syrk_tc_, syrk_tc);
+  up(t_inv_sn_, inv_sn);
+  up(t_inv_step_, inv_step);
 }
 
 void CholEngine::build_solve()
@@ -553,11 +573,19 @@ void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
         dev::potrf_inv(cd_, L.potrf[j].cnt, t_potrf_.p + L.potrf[j].off, step, L.potrf[j].maxb, b,
                        st_);
       });
-      timed(L.trsm[j].short_rows ? PK_DENSE_NT_SHORT : PK_DENSE_NT, L.trsm[j], [&] {
-        dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
-                      t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, b, st_,
-                      L.trsm[j].short_rows);
-      });
+      if (L.potrf[j].maxb > 64) {
+        // 128-wide blocks: blocked substitution, no whole inverse needed yet
+        timed(L.trsm[j].short_rows ? PK_TRSM_SHORT : PK_TRSM, L.trsm[j], [&] {
+          dev::trsm(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
+                    t_trsm_tr_.p + L.trsm[j].off, step, b, st_, L.trsm[j].short_rows);
+        });
+      } else {
+        timed(L.trsm[j].short_rows ? PK_DENSE_NT_SHORT : PK_DENSE_NT, L.trsm[j], [&] {
+          dev::dense_nt(cd_, L.trsm[j].cnt, t_trsm_sn_.p + L.trsm[j].off,
+                        t_trsm_tr_.p + L.trsm[j].off, t_trsm_tc_.p + L.trsm[j].off, 2, step, b,
+                        st_, L.trsm[j].short_rows);
+        });
+      }
       if (rest_pending && upd_hi >= rest_lo) {
         HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
         rest_pending = false;
@@ -582,6 +610,9 @@ void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
       }
     }
     if (rest_pending) HIPCHK(hipStreamWaitEvent(st_, ev_rest_, 0));
+    timed(PK_INV_BLOCKS, L.inv, [&] {
+      dev::inv_blocks(cd_, L.inv.cnt, t_inv_sn_.p + L.inv.off, t_inv_step_.p + L.inv.off, b, st_);
+    });
     timed(PK_DENSE_NT, L.syrk, [&] {
       dev::dense_nt(cd_, L.syrk.cnt, t_syrk_sn_.p + L.syrk.off, t_syrk_tr_.p + L.syrk.off,
                     t_syrk_tc_.p + L.syrk.off, 1, 0, b, st_);

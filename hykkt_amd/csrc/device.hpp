@@ -171,7 +171,7 @@ class CholEngine {
     double flops = 0.0, bytes = 0.0;   // algorithmic, for the profiler
   };
   struct Level {
-    Launch ea_panel, ea_u, syrk;
+    Launch ea_panel, ea_u, syrk, inv;
     std::vector<Launch> potrf, trsm, upd_next, upd_rest;
     // per step: its position in its group of consecutive steps and the
     // group's size (1: the step updates after its own block; see build_tasks)
@@ -205,6 +205,7 @@ class CholEngine {
   DevBuf<int> t_ea_p_, t_ea_col_, t_ea_cptr_, t_ea_cc_, t_ea_ccol_;
   DevBuf<int> t_potrf_, t_trsm_sn_, t_trsm_tr_, t_trsm_tc_, t_upd_sn_, t_upd_tr_, t_upd_tc_;
   DevBuf<int> t_syrk_sn_, t_syrk_tr_, t_syrk_tc_;
+  DevBuf<int> t_inv_sn_, t_inv_step_;
   int n_cu_ = 1;   // compute units: persistent-launch width
   // solve items
   DevBuf<int> fwd_items_, bwd_items_, pull_ptr_, pull_src_, flags_;

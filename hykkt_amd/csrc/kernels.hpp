@@ -213,8 +213,19 @@ void extend_add(const CholDev& c, int ntasks, const int* task_p, const int* task
 // Factor + invert the (<= 128 wide) diagonal block of step `step` of each
 // supernode; max_b = largest block in the launch (selects the LDS footprint).
 // dinv slots must be zero-initialized once (only the b x b lower part is written).
+// Blocks wider than 64 (max_b > 64) leave only the inverses of their 16x16
+// diagonal blocks in the slot: their TRSM is trsm(), the whole inverse comes
+// from inv_blocks() after the level.
 void potrf_inv(const CholDev& c, int nsn, const int* sns, int step, int max_b, const Batch& b,
                hipStream_t st);
+// L21 = A21 L_kk^{-T} below the 128-wide diagonal blocks of step `step` by blocked
+// substitution (tasks: supernode, tile row of TM = 32 (short_rows) or 128 rows).
+void trsm(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr, int step,
+          const Batch& b, hipStream_t st, bool short_rows);
+// The whole inverses of those blocks (tasks: supernode, step), same bits as formed
+// inside potrf_inv.
+void inv_blocks(const CholDev& c, int ntasks, const int* task_sn, const int* task_step,
+                const Batch& b, hipStream_t st);
 
 // MFMA tile kernel over (supernode, tile row, tile col) tasks.  mode 0: trailing
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the

@@ -405,9 +405,45 @@ __device__ __forceinline__ void lower_entry(int e, int& r, int& col)
   r = first ? pr + o : col + o - (BK - pr);
 }
 
+// The doubling levels and the store of the whole inverse, from P holding L
+// in its lower part and the inverses of the 16x16 diagonal blocks (see
+// potrf_block), into the 128 x 128 dinv slot W (column-major; entries
+// outside the b x b lower triangle stay at the zeros written when the slot
+// was created).  All threads of the workgroup.
+template <int BK>
+__device__ __forceinline__ void inverse_full(double (*P)[BK + 1], double* W, int b, int tid,
+                                             int wave, int lane)
+{
+  constexpr int NL = BK * (BK + 1) / 2, PER = (NL + 255) / 256;
+  if constexpr (BK >= 32) inverse_level<BK, 16>(P, wave, lane);
+  if constexpr (BK >= 64) inverse_level<BK, 32>(P, wave, lane);
+  if constexpr (BK >= 128) inverse_level<BK, 64>(P, wave, lane);
+  // over the folded lower triangle, LDS reads first so they overlap
+  constexpr int SB2 = (PER + 1) / 2;   // entries per batch
+#pragma unroll
+  for (int u0 = 0; u0 < PER; u0 += SB2) {
+    double xv[SB2];
+#pragma unroll
+    for (int u = 0; u < SB2; u++) {
+      int r, col;
+      lower_entry<BK>(min(tid + (u0 + u) * 256, NL - 1), r, col);
+      xv[u] = xinv<BK>(P, r, col);
+    }
+#pragma unroll
+    for (int u = 0; u < SB2; u++) {
+      int r, col;
+      const int e = tid + (u0 + u) * 256;
+      lower_entry<BK>(min(e, NL - 1), r, col);
+      if (u0 + u < PER && e < NL && r < b) W[static_cast<int64_t>(col) * BT + r] = xv[u];
+    }
+  }
+}
+
 // Diagonal block `step` of supernode s, by one workgroup of 256 threads with
 // smem >= potrf_smem<BK>().  BK: padded block edge (32, 64 or 128) >= b.
-template <int BK, bool COH>
+// FULL: the whole inverse; otherwise only the inverses of its 16x16 diagonal
+// blocks (the blocked TRSM's), the rest formed after the level (k_inv_blocks).
+template <int BK, bool FULL>
 __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double* dinv, int* pivot,
                                             int s, int step, double* smem)
 {
@@ -434,7 +470,7 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
       int r, col;
       lower_entry<BK>(min(tid + u * 256, NL - 1), r, col);
       const bool in = r < b;
-      const double t = ldg<COH>(D + (in ? r + static_cast<int64_t>(col) * si.f : 0));
+      const double t = ldg<false>(D + (in ? r + static_cast<int64_t>(col) * si.f : 0));
       v[u] = in ? t : (r == col ? 1.0 : 0.0);
     }
 #pragma unroll
@@ -558,7 +594,7 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
     const int e = tid + u * 256;
     lower_entry<BK>(min(e, NL - 1), r, col);
     const double lv = P[r][col];
-    if (e < NL && r < b) stg<COH>(D + r + static_cast<int64_t>(col) * si.f, lv);
+    if (e < NL && r < b) stg<false>(D + r + static_cast<int64_t>(col) * si.f, lv);
   }
 
   // ---- inverse: the 16x16 diagonal blocks (16 lanes each), then doubling ----
@@ -582,30 +618,17 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
     }
   }
   __syncthreads();
-  if constexpr (BK >= 32) inverse_level<BK, 16>(P, wave, lane);
-  if constexpr (BK >= 64) inverse_level<BK, 32>(P, wave, lane);
-  if constexpr (BK >= 128) inverse_level<BK, 64>(P, wave, lane);
-
-  // the inverse into the 128 x 128 dinv slot (column-major; entries outside
-  // the b x b lower triangle stay at the zeros written when the slot was
-  // created), over the folded lower triangle, LDS reads first so they overlap
   double* W = dinv + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
-  constexpr int SB2 = (PER + 1) / 2;   // entries per batch
-#pragma unroll
-  for (int u0 = 0; u0 < PER; u0 += SB2) {
-    double xv[SB2];
-#pragma unroll
-    for (int u = 0; u < SB2; u++) {
+  if constexpr (FULL) {
+    inverse_full<BK>(P, W, b, tid, wave, lane);
+  } else {
+    // the 16x16 diagonal blocks of the inverse, at their places in the slot
+    constexpr int NE = (BK / PW) * (PW * (PW + 1) / 2);
+    for (int e = tid; e < NE; e += 256) {
+      const int bb = (e / (PW * (PW + 1) / 2)) * PW;
       int r, col;
-      lower_entry<BK>(min(tid + (u0 + u) * 256, NL - 1), r, col);
-      xv[u] = xinv<BK>(P, r, col);
-    }
-#pragma unroll
-    for (int u = 0; u < SB2; u++) {
-      int r, col;
-      const int e = tid + (u0 + u) * 256;
-      lower_entry<BK>(min(e, NL - 1), r, col);
-      if (u0 + u < PER && e < NL && r < b) stg<COH>(W + static_cast<int64_t>(col) * BT + r, xv[u]);
+      lower_entry<PW>(e % (PW * (PW + 1) / 2), r, col);
+      if (bb + r < b) W[static_cast<int64_t>(bb + col) * BT + bb + r] = xinv<BK>(P, bb + r, bb + col);
     }
   }
   if (tid == 0 && bad != INT_MAX) atomicMin(pivot, bad);
@@ -620,8 +643,199 @@ __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restr
   if (sy < 0) return;
   const int s = sns[blockIdx.x];
   if (step * BT >= c.sn_first[s + 1] - c.sn_first[s]) return;
-  potrf_block<BK, false>(c, batch_at(c.L, sy, bt), batch_at(c.dinv, sy, bt), c.pivot + sy, s,
-                         step, smem);
+  potrf_block<BK, (BK < 128)>(c, batch_at(c.L, sy, bt), batch_at(c.dinv, sy, bt), c.pivot + sy,
+                              s, step, smem);
+}
+
+// The whole inverse of 128-wide diagonal blocks whose factorization left only
+// the 16x16 diagonal blocks' inverses (potrf_block<128, false>): L_kk from
+// the panel, those inverses from the dinv slot, then the doubling levels.
+// Task (s, step); system blockIdx.y.  Same arithmetic as the in-kernel
+// inverse, so the same bits.
+__global__ void __launch_bounds__(256) k_inv_blocks(CholDev c, const int* __restrict__ tsn,
+                                                    const int* __restrict__ tstep, Batch bt)
+{
+  constexpr int BK = BT;
+  constexpr int NL = BK * (BK + 1) / 2, PER = (NL + 255) / 256;
+  constexpr int T16 = PW * (PW + 1) / 2;
+  extern __shared__ double smem[];
+  const int sy = batch_sys(bt);
+  if (sy < 0) return;
+  const int s = tsn[blockIdx.x], step = tstep[blockIdx.x];
+  const SnInfo si = sninfo(c, batch_at(c.L, sy, bt), s);
+  const int k0 = step * BT;
+  if (k0 >= si.nc) return;
+  const int b = min(BK, si.nc - k0);
+  double (*P)[BK + 1] = reinterpret_cast<double (*)[BK + 1]>(smem);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double* D = si.Ls + static_cast<int64_t>(k0) * si.f + k0;
+  double* W = batch_at(c.dinv, sy, bt) + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
+  {
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      int r, col;
+      lower_entry<BK>(min(tid + u * 256, NL - 1), r, col);
+      const bool in = r < b;
+      const double t = D[in ? r + static_cast<int64_t>(col) * si.f : 0];
+      v[u] = in ? t : (r == col ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      int r, col;
+      lower_entry<BK>(min(tid + u * 256, NL - 1), r, col);
+      if (tid + u * 256 < NL) P[r][col] = v[u];
+    }
+  }
+  for (int e = tid; e < (BK / PW) * T16; e += 256) {
+    const int bb = (e / T16) * PW;
+    int r, col;
+    lower_entry<PW>(e % T16, r, col);
+    const double x = (bb + r < b) ? W[static_cast<int64_t>(bb + col) * BT + bb + r]
+                                  : (r == col ? 1.0 : 0.0);
+    if (r == col) P[bb + col][BK] = x;
+    else P[bb + col][bb + r] = x;
+  }
+  __syncthreads();
+  inverse_full<BK>(P, W, b, tid, wave, lane);
+}
+
+// L21 = A21 L_kk^{-T} for the rows below 128-wide diagonal block `step`, by
+// blocked forward substitution over its eight 16-column panels (no inverse of
+// the whole block needed): X_p^T = inv16(L_pp) (A_p^T - sum_{q<p} L_pq X_q^T).
+// A wave owns 16-row groups; a group's X_q^T live in MFMA accumulators, whose
+// lane layout (element v = D[4v + l/16][l%16]) is the B operand's of k-step
+// v, so they feed the later panels' products without going through LDS.  The
+// 28 blocks L_pq (p > q) and the 8 inverses inv16(L_pp) are staged in LDS
+// first (each 16 x 16, column-major, in batches of loads), each panel's A
+// values are loaded while the one before is computed.  Few registers: a
+// 32-row tile fits beside two workgroups of the trailing update on one CU,
+// so the critical path's TRSM does not wait for them to drain.  TM rows per tile (32 or 128: 2 or 8
+// groups); every row takes the same operations in the same order whatever
+// the tile, so both shapes give the same bits.  In place.  System blockIdx.y.
+constexpr int TRSM_NB = (BT / PW) * (BT / PW - 1) / 2;   // blocks L_pq, p > q
+constexpr size_t TRSM_SMEM = (TRSM_NB + BT / PW) * PW * PW * sizeof(double);
+
+template <int TM>
+__global__ void __launch_bounds__(256) k_trsm(CholDev c, const int* __restrict__ tsn,
+                                              const int* __restrict__ ttr, int step, Batch bt)
+{
+  constexpr int NG = TM / 16, NGW = (NG + 3) / 4;   // groups, groups per wave
+  constexpr int NPN = BT / PW;                       // panels
+  extern __shared__ double smem[];
+  double* Ls = smem;                                 // block (p, q) at (p(p-1)/2 + q) * 256
+  double* Is = smem + TRSM_NB * PW * PW;             // inv16(L_pp) at p * 256
+  const int sy = batch_sys(bt);
+  if (sy < 0) return;
+  const int s = tsn[blockIdx.x], tr = ttr[blockIdx.x];
+  const SnInfo si = sninfo(c, batch_at(c.L, sy, bt), s);
+  const int k0 = step * BT;
+  const int b = min(BT, si.nc - k0);
+  const int R0 = k0 + b + tr * TM;
+  const int M = min(TM, si.f - R0);
+  if (b <= 0 || M <= 0) return;   // uniform over the workgroup
+  const int64_t f = si.f;
+  const double* Lkk = si.Ls + static_cast<int64_t>(k0) * f + k0;   // L(k0 + r, k0 + c) at c*f + r
+  double* A = si.Ls + static_cast<int64_t>(k0) * f + R0;            // A(row, col) at col*f + row
+  const double* W = batch_at(c.dinv, sy, bt) + c.dinv_off[s] + static_cast<int64_t>(step) * BT * BT;
+  const int np = (b + PW - 1) / PW;
+  // stage: thread t holds entry (row t % 16, column t / 16) of every block;
+  // rows at or past b (the panel rows below a partial block) read as 0.  All
+  // loads are issued before the first LDS store.
+  {
+    const int rr = threadIdx.x % PW, cc = threadIdx.x / PW;
+    constexpr int NBAT = 9;   // loads in flight per batch
+#pragma unroll
+    for (int t0 = 0; t0 < TRSM_NB + NPN; t0 += NBAT) {
+    double v[NBAT];
+#pragma unroll
+    for (int t = t0; t < t0 + NBAT; t++) {
+      if (t < TRSM_NB) {
+        int p = 1;
+        while ((p + 1) * p / 2 <= t) p++;   // compile-time after unrolling
+        const int q = t - p * (p - 1) / 2;
+        const int row = PW * p + rr;
+        const bool in = p < np && row < b;
+        const double w = Lkk[in ? static_cast<int64_t>(PW * q + cc) * f + row : 0];
+        v[t - t0] = in ? w : 0.0;
+      } else {
+        const int p = t - TRSM_NB;
+        v[t - t0] = W[static_cast<int64_t>(PW * p + cc) * BT + PW * p + rr];
+      }
+    }
+#pragma unroll
+    for (int t = t0; t < t0 + NBAT; t++) smem[t * PW * PW + threadIdx.x] = v[t - t0];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i16 = lane & 15, k4 = lane >> 4;
+  int grow[NGW];   // first row of each of this wave's groups (-1: none)
+#pragma unroll
+  for (int gi = 0; gi < NGW; gi++) {
+    const int g = wave + 4 * gi;
+    grow[gi] = (g < NG && g * 16 < M) ? g * 16 : -1;
+  }
+  if (grow[0] < 0) return;   // no barrier follows
+  // A_p^T of each group: element v = A(row i16, column 16p + 4v + k4)
+  auto load_a = [&](int p, d4 (&a)[NGW]) {
+#pragma unroll
+    for (int gi = 0; gi < NGW; gi++) {
+      const int row = grow[gi] + i16;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int col = PW * p + 4 * v + k4;
+        a[gi][v] = (grow[gi] >= 0 && row < M && col < b) ? A[col * f + row] : 0.0;
+      }
+    }
+  };
+  d4 x[NGW][NPN];
+  d4 an[NGW];
+  load_a(0, an);
+#pragma unroll
+  for (int p = 0; p < NPN; p++) {
+    if (p >= np) break;
+    d4 acc[NGW];
+#pragma unroll
+    for (int gi = 0; gi < NGW; gi++) acc[gi] = an[gi];
+    if (p + 1 < np) load_a(p + 1, an);
+    // - sum_{q<p} L_pq X_q^T: A operand -L(16p + i16, 16q + 4 s4 + k4)
+#pragma unroll
+    for (int q = 0; q < NPN; q++) {
+      if (q >= p) break;
+      const double* blk = Ls + (p * (p - 1) / 2 + q) * PW * PW;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; s4++) {
+        const double a = -blk[(4 * s4 + k4) * PW + i16];
+#pragma unroll
+        for (int gi = 0; gi < NGW; gi++)
+          acc[gi] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[gi][q][s4], acc[gi], 0, 0, 0);
+      }
+    }
+    // X_p^T = inv16(L_pp) acc: A operand inv16(i16, 4 s4 + k4)
+    d4 r[NGW];
+#pragma unroll
+    for (int gi = 0; gi < NGW; gi++) r[gi] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      const double a = Is[p * PW * PW + (4 * s4 + k4) * PW + i16];
+#pragma unroll
+      for (int gi = 0; gi < NGW; gi++)
+        r[gi] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[gi][s4], r[gi], 0, 0, 0);
+    }
+#pragma unroll
+    for (int gi = 0; gi < NGW; gi++) {
+      if (p + 1 < NPN) x[gi][p] = r[gi];   // the last panel's X is not read again
+      const int row = grow[gi] + i16;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int col = PW * p + 4 * v + k4;
+        if (grow[gi] >= 0 && row < M && col < b) A[col * f + row] = r[gi][v];
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -636,6 +850,41 @@ void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, cons
     k_dense_nt<32><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
   else
     k_dense_nt<128><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
+  HIPCHK(hipGetLastError());
+}
+
+void inv_blocks(const CholDev& c, int ntasks, const int* tsn, const int* tstep, const Batch& b,
+                hipStream_t st)
+{
+  if (ntasks <= 0 || b.count <= 0) return;
+  constexpr size_t SM = (BT * (BT + 1) + BT) * sizeof(double);
+  static const bool attr = [] {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_inv_blocks),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(SM)));
+    return true;
+  }();
+  (void)attr;
+  k_inv_blocks<<<dim3(ntasks, b.count), 256, SM, st>>>(c, tsn, tstep, b);
+  HIPCHK(hipGetLastError());
+}
+
+void trsm(const CholDev& c, int ntasks, const int* tsn, const int* ttr, int step, const Batch& b,
+          hipStream_t st, bool short_rows)
+{
+  if (ntasks <= 0 || b.count <= 0) return;
+  static const bool attr = [] {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_trsm<32>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(TRSM_SMEM)));
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_trsm<128>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(TRSM_SMEM)));
+    return true;
+  }();
+  (void)attr;
+  const dim3 grid(ntasks, b.count);
+  if (short_rows) k_trsm<32><<<grid, 256, TRSM_SMEM, st>>>(c, tsn, ttr, step, b);
+  else k_trsm<128><<<grid, 256, TRSM_SMEM, st>>>(c, tsn, ttr, step, b);
   HIPCHK(hipGetLastError());
 }
 

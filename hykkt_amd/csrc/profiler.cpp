@@ -18,7 +18,8 @@ const char* Profiler::name(int k)
       "k_dense_nt<128>",      "k_dense_nt<32>",      "k_potrf_inv",         "k_extend_add",
       "k_fwd_forest",         "k_solve_fwd<1024, true>", "k_solve_bwd<1024, true>",
       "k_solve_bwd<256, false>", "k_spmv",             "k_spgemm_values",
-      "k_spgemm_scatter"};
+      "k_spgemm_scatter",     "k_trsm<128>",         "k_trsm<32>",
+      "k_inv_blocks"};
   return (k >= 0 && k < PK_COUNT) ? names[k] : "";
 }
 

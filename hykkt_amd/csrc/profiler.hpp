@@ -24,6 +24,9 @@ enum ProfKind {
   PK_SPMV,
   PK_SPGEMM_VALUES,
   PK_SPGEMM_SCATTER,
+  PK_TRSM,               // blocked-substitution TRSM, 128-row tiles
+  PK_TRSM_SHORT,         // the same, 32-row tiles
+  PK_INV_BLOCKS,
   PK_COUNT
 };
 

@@ -23,11 +23,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # Hardware queues for this process's streams: the served stream runs one
-# solver slot (two streams) per resident pattern concurrently; with HIP's
-# default of 4 queues, slots share queues and their kernels serialise.  Set
+# solver slot (two streams) per resident pattern concurrently, 16 slots; with
+# fewer queues than streams, slots share queues and their kernels serialise
+# (at 16 queues the served stream ran at 550 solves/s, at 32 at 987).  Set
 # before the HIP runtime starts; the plain path's throughput is the same
-# with 4 or 16.
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# with 4 or 32.
+os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 
 def metric_name():

@@ -66,8 +66,16 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
   // the solve is skipped.
   dev::copy(o.p, b, m, live, o.st);
   dev::copy(o.s, b, m, live, o.st);
-  if (x0_zero) dev::copy(o.r, b, m, live, o.st);
-  else apply_s(o, o.x, -1.0, 1.0, b, o.r, live);
+  if (o.tq) {
+    dev::fill(o.tq, 0.0, o.n, live, o.st);
+    dev::fill(o.tz, 0.0, o.n, live, o.st);
+  }
+  if (x0_zero) {
+    dev::copy(o.r, b, m, live, o.st);
+  } else {
+    apply_s(o, o.x, -1.0, 1.0, b, o.r, live);
+    if (o.tz) dev::copy(o.tz, o.t1, o.n, live, o.st);   // A^{-1} Jh^T x0
+  }
   // gam = r.r; r.r == 0: x0 already solves the system (the reference would
   // divide 0/0 here) and the system stops with 0 iterations
   dev::dot_partial(m, o.r, o.r, o.part, live, o.st);
@@ -86,6 +94,7 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
   int next_check = std::max(1, chunk);
   for (int i = 0; i < itmax; i++) {
     dev::cg_update_dot(m, o.sc, o.p, o.s, o.x, o.r, o.w, o.part, live, o.st);
+    if (o.tq) dev::cg_track(o.n, o.sc, o.tq, o.tz, o.t1, live, o.st);
     dev::dot_final(dev::reduce_blocks(m), o.part, o.sc, 1, true, false, i, tol, itmax, fl, live,
                    o.st);
     apply_s(o, o.r, 1.0, 0.0, nullptr, o.w, live, o.r);

@@ -234,6 +234,9 @@ struct SchurOps {
   double *t1 = nullptr;                                            // n
   double *x = nullptr, *r = nullptr, *w = nullptr, *p = nullptr, *s = nullptr;   // m
   double *sc = nullptr, *part = nullptr;   // 8 scalars; partials (max of both reductions)
+  // optional (n each): z = A^{-1} Jh^T x carried along with the iterate x
+  // (q the same image of the direction p), so the recovery needs no solve
+  double *tq = nullptr, *tz = nullptr;
   int *live = nullptr, *iters = nullptr, *status = nullptr;   // per system
   double* gath = nullptr;   // 2 doubles per system: scalars gathered for the readback
   Profiler* prof = nullptr;  // optional launch timing of the SpMVs

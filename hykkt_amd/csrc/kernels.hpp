@@ -47,6 +47,12 @@ void gather_scale(double* dst, const double* src, const int* map, const double* 
 // y[i] *= s[i]  (vec_scale, matrix_vector_ops.cu:385-392)
 void vec_scale(double* y, const double* s, int n, const Batch& b, hipStream_t st);
 // out = a + b
+// out = a - b (per system)
+void sub(double* out, const double* a, const double* bv, int64_t n, const Batch& b,
+         hipStream_t st);
+// CG tracking: q = beta q + t, z += alpha q (alpha, beta from the CG scalars sc)
+void cg_track(int n, const double* sc, double* q, double* z, const double* t, const Batch& b,
+              hipStream_t st);
 void add(double* out, const double* a, const double* bv, int64_t n, const Batch& b,
          hipStream_t st);
 // out[k*nslot + j] = base[k*vs + j] for j < nslot and each system k (b.count

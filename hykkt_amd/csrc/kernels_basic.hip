@@ -172,6 +172,19 @@ __global__ void k_vec_scale(double* __restrict__ y, const double* __restrict__ s
     y[i] *= sv[i];
 }
 
+__global__ void k_sub(double* __restrict__ out, const double* __restrict__ a,
+                      const double* __restrict__ bv, int64_t n, Batch b)
+{
+  const int s = sys_of(b);
+  if (s < 0) return;
+  out = at(out, s, b);
+  a = at(a, s, b);
+  bv = at(bv, s, b);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - bv[i];
+}
+
 __global__ void k_add(double* __restrict__ out, const double* __restrict__ a,
                       const double* __restrict__ bv, int64_t n, Batch b)
 {
@@ -516,6 +529,27 @@ __global__ void k_cg_update_dot(int m, const double* __restrict__ sc, double* __
   }
   const double tot = block_sum(t);
   if (threadIdx.x == 0) at(part, s, b)[blockIdx.x] = tot;
+}
+
+// The CG's image of its iterate under A^{-1} Jh^T, carried along with it:
+// q = beta q + t (t = A^{-1} Jh^T r, formed by the last Schur product),
+// z += alpha q, the same coefficients in the same forms as p and x, so z =
+// A^{-1} Jh^T x without a solve.
+__global__ void k_cg_track(int n, const double* __restrict__ sc, double* __restrict__ q,
+                           double* __restrict__ z, const double* __restrict__ t, Batch b)
+{
+  const int s = sys_of(b);
+  if (s < 0) return;
+  const double* c = at(sc, s, b);
+  q = at(q, s, b);
+  z = at(z, s, b);
+  t = at(t, s, b);
+  const double alpha = c[3], beta = c[4];
+  for (int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+    const double qi = beta * q[i] + t[i];
+    q[i] = qi;
+    z[i] = fma(alpha, qi, z[i]);
+  }
 }
 
 // ------------------------------------------------------------- recover / residual
@@ -863,6 +897,22 @@ void dot_partial(int n, const double* a, const double* bv, double* part, const B
 {
   if (b.count <= 0) return;
   k_dot_partial<<<dim3(reduce_blocks(n), b.count), TPB, 0, st>>>(n, a, bv, part, b);
+  LAUNCH_CHECK();
+}
+
+void sub(double* out, const double* a, const double* bv, int64_t n, const Batch& b,
+         hipStream_t st)
+{
+  if (n <= 0 || b.count <= 0) return;
+  k_sub<<<grid2(n, b), TPB, 0, st>>>(out, a, bv, n, b);
+  LAUNCH_CHECK();
+}
+
+void cg_track(int n, const double* sc, double* q, double* z, const double* t, const Batch& b,
+              hipStream_t st)
+{
+  if (n <= 0 || b.count <= 0) return;
+  k_cg_track<<<dim3(reduce_blocks(n), b.count), TPB, 0, st>>>(n, sc, q, z, t, b);
   LAUNCH_CHECK();
 }
 

@@ -26,6 +26,7 @@ struct Solver::Slots {
   int64_t jd_vs = 0, ryd_s = 0, rx_til = 0, htil = 0, jcs = 0, ry_s = 0, max_d = 0, scale = 0;
   int64_t jch_v = 0, jcth_v = 0, rxp = 0, tmp_n = 0, t1 = 0;
   int64_t cx = 0, cr = 0, cw = 0, cp = 0, cs = 0, schur = 0, sc = 0, part = 0;
+  int64_t h0 = 0, cq = 0, cz = 0;
   int64_t dx = 0, dsv = 0, dyc = 0, dyd = 0, sq_n = 0, sq_m = 0, sq_d2 = 0, sq_d4 = 0, res = 0;
   int64_t e_x = 0, e_s = 0, e_y = 0, e_d = 0, e_rxt = 0, e_ysc = 0, e_rxp = 0, e_schur = 0;
   int64_t e_cx = 0, c_dx = 0, c_dsv = 0, c_dyc = 0, c_dyd = 0;
@@ -406,6 +407,9 @@ void Solver::analyze()
   z.rxp = slot(n);
   z.tmp_n = slot(n);
   z.t1 = slot(n);
+  z.h0 = slot(n);
+  z.cq = slot(n);
+  z.cz = slot(n);
   z.cx = slot(m);
   z.cr = slot(m);
   z.cw = slot(m);
@@ -893,6 +897,7 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   });
   dev::copy(P(z.t1), P(w.rxp), n, b, st_);
   S.chol->solve(P(z.t1), b);
+  dev::copy(P(z.h0), P(z.t1), n, b, st_);   // A^{-1} r^x, kept for the recovery
   timed(PK_SPMV, fl, by_j, b, [&] {
     dev::spmv(m, S.jch_rp.p, S.jch_ci.p, nullptr, P(z.jch_v), 1.0, nullptr, P(z.t1), -1.0,
               P(ysc), P(w.schur), S.jch_blk.p, S.jch_nblk, b, st_);
@@ -925,6 +930,8 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   ops.iters = S.flag(FL_CG_IT);
   ops.status = S.flag(FL_CG_ST);
   ops.gath = S.gath.p;
+  ops.tq = P(z.cq);
+  ops.tz = P(z.cz);
   ops.prof = &prof;
   ops.nnz = S.nnz_jc;
   S.cg_host.ensure(4 * static_cast<size_t>(b.count));
@@ -939,11 +946,9 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
       schur_cg_run(ops, P(w.schur), opts.cg_tol, opts.cg_itmax, b, !x0, chunk, chunk_next);
   if (after_cg) HIPCHK(hipEventRecord(after_cg, st_));
 
-  timed(PK_SPMV, fl, by_t, b, [&] {
-    dev::spmv(n, S.jcth_rp.p, S.jcth_ci.p, nullptr, P(z.jcth_v), -1.0, nullptr, P(w.cx), 1.0,
-              P(w.rxp), P(z.t1), S.jcth_blk.p, S.jcth_nblk, b, st_);
-  });
-  S.chol->solve(P(z.t1), b);
+  // recover_solution: dx = A^{-1} (r^x - Jc^T y) = A^{-1} r^x - A^{-1} Jc^T y, the
+  // second term carried along by the CG (no third solve)
+  dev::sub(P(z.t1), P(z.h0), P(z.cz), n, b, st_);
   dev::gather_scale(P(dx), P(z.t1), S.ipos_d.p, P(z.max_d), n, b, st_);
   dev::copy(P(dyc), P(w.cx), m, b, st_);
   dev::vec_scale(P(dyc), P(z.max_d) + n, m, b, st_);

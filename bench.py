@@ -25,10 +25,14 @@ sys.path.insert(0, ROOT)
 # Hardware queues for this process's streams: the served stream runs one
 # solver slot (two streams) per resident pattern concurrently, 16 slots; with
 # fewer queues than streams, slots share queues and their kernels serialise
-# (at 16 queues the served stream ran at 550 solves/s, at 32 at 987).  Set
-# before the HIP runtime starts; the plain path's throughput is the same
-# with 4 or 32.
-os.environ["GPU_MAX_HW_QUEUES"] = "32"
+# (at 16 queues the served stream ran at 585 solves/s, at 32 at 1,127).  The
+# plain path, one solve at a time, is slower with many queues (its short
+# launches wait longer: 191 solves/s at 4 or 16, 96 at 32), so the plain legs of
+# the served and batched comparisons run in a child process at HIP's default
+# of 4 (PLAIN_QUEUES): each form in the configuration it is deployed in.  Set
+# before the HIP runtime starts.
+PLAIN_QUEUES = "4"
+os.environ["GPU_MAX_HW_QUEUES"] = PLAIN_QUEUES if "--plain-legs" in sys.argv else "32"
 
 
 def metric_name():
@@ -144,13 +148,13 @@ def cpu_baseline(g, perm, nsample, gpu_solutions):
             "max_resid4": worst, "gpu_err_vs_oracle": errs}
 
 
-def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0):
+def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0, legs=("served", "plain")):
     """The served form (continuous batching over 16 resident patterns of 1k,
     5k and 20k buses, 6:3:1) and the plain path (one request at a time through
     the drop-in solver of each pattern) on the same Poisson request stream.
     Both are over capacity at 2,000 requests/s: the figures are saturation
     throughput and queueing latency over `seconds`, p50/p99 over the requests
-    that arrived after `warmup` seconds."""
+    that arrived after `warmup` seconds.  `legs`: which of the two run here."""
     from hykkt_amd.served import Server, make_patterns, poisson_requests, summarize
     t0 = time.perf_counter()
     pats = make_patterns(npat, seed=100)
@@ -160,15 +164,13 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0):
           f"{srv.hbm_used / 1e9:.1f} GB, capacities {srv.capacity}", file=sys.stderr, flush=True)
     reqs = poisson_requests(pats, rate, seconds, seed=7, served=srv.served)
     res = {}
-    for mode in ("served", "plain"):
+    for mode in legs:
         res[mode] = summarize(*srv.run(reqs, plain=(mode == "plain"), deadline=seconds),
                               patterns=pats, solo_s=srv.solo_s, warmup=warmup)
         print(f"bench: {mode} stream: {res[mode]}", file=sys.stderr, flush=True)
     srv.close()
     for p in pats:
         p.free()
-    sp, pp = res["served"]["solves_per_s"], res["plain"]["solves_per_s"]
-    sw, pw = res["served"]["solo_s_per_s"], res["plain"]["solo_s_per_s"]
     return {"stream": f"Poisson {rate:.0f} requests/s for {seconds:.0f} s over {npat} resident "
                       "patterns (1k/5k/20k buses, 6:3:1); a pattern's i-th request is its "
                       f"interior-point step i, values from a pool of {len(pats[0].pool)} fresh "
@@ -177,17 +179,28 @@ def served_vs_plain(rate, seconds, npat=16, budget=200e9, warmup=5.0):
             "capacity_by_pattern": {str(pats[p].nbus) + "/" + str(p): c
                                     for p, c in srv.capacity.items()},
             "patterns_served": len(srv.served), "hbm_used_gb": srv.hbm_used / 1e9,
-            "setup_s": setup, "served": res["served"], "plain": res["plain"],
-            "solo_ms": {str(pats[p].nbus) + "/" + str(p): 1e3 * t for p, t in srv.solo_s.items()},
-            "served_over_plain": sp / pp if pp > 0 else None,
-            "served_over_plain_solo_weighted": sw / pw if pw > 0 else None}
+            "setup_s": setup, "served": res.get("served"), "plain": res.get("plain"),
+            "solo_ms": {str(pats[p].nbus) + "/" + str(p): 1e3 * t for p, t in srv.solo_s.items()}}
 
 
-def batched_vs_plain():
+def ratios(served, batched):
+    """Served over plain and batched over plain, once both legs are in."""
+    if served and served.get("served") and served.get("plain"):
+        sp, pp = served["served"]["solves_per_s"], served["plain"]["solves_per_s"]
+        sw, pw = served["served"]["solo_s_per_s"], served["plain"]["solo_s_per_s"]
+        served["served_over_plain"] = sp / pp if pp else None
+        served["served_over_plain_solo_weighted"] = sw / pw if pw else None
+    if batched and batched.get("batched") and batched.get("plain"):
+        # systems within the residual bound per second, batched over plain
+        batched["batched_over_plain"] = (batched["batched"]["solves_per_s"] /
+                                         batched["plain"]["solves_per_s"])
+
+
+def batched_vs_plain(legs=("batched", "plain")):
     """The batched form (256 systems: 32 patterns of 200..5000 buses x 8 value
     draws, a gamma per system; each pattern's 8 draws in one pass of the batch
     engine, the patterns' passes concurrent) against the same systems one at a
-    time through the drop-in path."""
+    time through the drop-in path.  `legs`: which of the two run here."""
     from hykkt_amd.batched import make_batch, make_server, run_batch, summarize_batch
     t0 = time.perf_counter()
     pats = make_batch(seed=3)
@@ -196,7 +209,7 @@ def batched_vs_plain():
     if len(srv.served) != len(pats):
         raise RuntimeError(f"batched: slots refused: {srv.refused}")   # over the budget
     res = {}
-    for mode in ("batched", "plain"):
+    for mode in legs:
         r, wall = run_batch(srv, pats, plain=(mode == "plain"))
         res[mode] = summarize_batch(r, wall, pats)
         print(f"bench: {mode} batch: {res[mode]}", file=sys.stderr, flush=True)
@@ -206,9 +219,22 @@ def batched_vs_plain():
     return {"batch": "256 systems: 32 patterns (200..5000 buses, branches 1.3x) x 8 value draws, "
                      "gamma log-uniform 1e4..1e6 per system, Ds log-uniform 1e-4..1e4",
             "hbm_used_gb": srv.hbm_used / 1e9, "setup_s": setup,
-            "batched": res["batched"], "plain": res["plain"],
-            # systems within the residual bound per second, batched over plain
-            "batched_over_plain": res["batched"]["solves_per_s"] / res["plain"]["solves_per_s"]}
+            "batched": res.get("batched"), "plain": res.get("plain")}
+
+
+def plain_legs(args):
+    """The plain legs of the served and batched comparisons, in a child
+    process at PLAIN_QUEUES hardware queues; their summaries (JSON on the
+    child's stdout, its progress lines on stderr)."""
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=PLAIN_QUEUES)
+    cmd = [sys.executable, os.path.abspath(__file__), "--plain-legs", "1",
+           "--served-seconds", str(args.served_seconds), "--served-rate", str(args.served_rate),
+           "--batched", str(args.batched)]
+    r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"plain legs: child exited with {r.returncode}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def main():
@@ -227,6 +253,9 @@ def main():
                     help="length of the served and plain request streams (0: skip)")
     ap.add_argument("--served-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served stream (requests/s)")
+    ap.add_argument("--plain-legs", type=int, default=0,
+                    help="(internal) run only the plain legs of the served and batched "
+                         "comparisons and print their summaries")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (3 times), the "
                          "known-size reference for HBM byte counters")
@@ -235,6 +264,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the KKT solve of one "
                                    "system does not shard across GPUs (see DESIGN.md)"}))
         sys.exit(2)
+
+    if args.plain_legs:
+        legs = {"served": None, "batched": None}
+        if args.served_seconds > 0:
+            legs["served"] = served_vs_plain(args.served_rate, args.served_seconds,
+                                             legs=("plain",))["plain"]
+        if args.batched:
+            legs["batched"] = batched_vs_plain(legs=("plain",))["plain"]
+        print(json.dumps(legs), flush=True)
+        return
 
     import hykkt_amd as H
     from hykkt_amd.generator import ds_spread_schedule, workload
@@ -382,9 +421,20 @@ def main():
     if args.served_seconds > 0 or args.batched:
         s.close()
     if args.served_seconds > 0:
-        out["served"] = served_vs_plain(args.served_rate, args.served_seconds)
+        out["served"] = served_vs_plain(args.served_rate, args.served_seconds, legs=("served",))
     if args.batched:
-        out["batched"] = batched_vs_plain()
+        out["batched"] = batched_vs_plain(legs=("batched",))
+    if args.served_seconds > 0 or args.batched:
+        pl = plain_legs(args)
+        note = (f"plain legs in a child process at GPU_MAX_HW_QUEUES={PLAIN_QUEUES} (HIP's "
+                "default), the served and batched legs at 32")
+        if out["served"]:
+            out["served"]["plain"] = pl["served"]
+            out["served"]["queues"] = note
+        if out["batched"]:
+            out["batched"]["plain"] = pl["batched"]
+            out["batched"]["queues"] = note
+        ratios(out["served"], out["batched"])
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, perm0, args.cpu_sample, gpu_sol)
         e = out["cpu_baseline"]["gpu_err_vs_oracle"]

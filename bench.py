@@ -22,17 +22,21 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# Hardware queues for this process's streams: the served stream runs one
-# solver slot (two streams) per resident pattern concurrently, 16 slots; with
-# fewer queues than streams, slots share queues and their kernels serialise
-# (at 16 queues the served stream ran at 585 solves/s, at 32 at 1,127).  The
-# plain path, one solve at a time, is slower with many queues (its short
-# launches wait longer: 191 solves/s at 4 or 16, 96 at 32), so the plain legs of
-# the served and batched comparisons run in a child process at HIP's default
-# of 4 (PLAIN_QUEUES): each form in the configuration it is deployed in.  Set
-# before the HIP runtime starts.
+# Hardware queues.  The served stream runs one solver slot (two streams) per
+# resident pattern concurrently, 16 slots; with fewer queues than streams, slots
+# share queues and their kernels serialise (at 16 queues the served stream ran
+# at 585 solves/s, at 32 at 1,127).  The plain path, one solve at a time, is
+# slower with many queues (its short launches wait longer: 191 solves/s at 4
+# or 16, 96 at 32), and so is any process while another process on the GPU
+# holds many queues.  So the served and batched legs run first, in a child
+# process at CONCURRENT_QUEUES, before this process starts the HIP runtime;
+# the headline sequence and the plain legs then run here at HIP's default of
+# PLAIN_QUEUES: each form in the configuration it is deployed in, alone on the
+# GPU.  Set before the HIP runtime starts.
 PLAIN_QUEUES = "4"
-os.environ["GPU_MAX_HW_QUEUES"] = PLAIN_QUEUES if "--plain-legs" in sys.argv else "32"
+CONCURRENT_QUEUES = "32"
+os.environ["GPU_MAX_HW_QUEUES"] = (CONCURRENT_QUEUES if "--concurrent-legs" in sys.argv
+                                   else PLAIN_QUEUES)
 
 
 def metric_name():
@@ -222,19 +226,30 @@ def batched_vs_plain(legs=("batched", "plain")):
             "batched": res.get("batched"), "plain": res.get("plain")}
 
 
-def plain_legs(args):
-    """The plain legs of the served and batched comparisons, in a child
-    process at PLAIN_QUEUES hardware queues; their summaries (JSON on the
-    child's stdout, its progress lines on stderr)."""
+def concurrent_legs(args):
+    """The served and batched legs of the two comparisons, in a child process
+    at CONCURRENT_QUEUES hardware queues; their summaries (JSON on the child's
+    stdout, its progress lines on stderr).  Called before this process starts
+    the HIP runtime, so the child has the GPU to itself."""
     import subprocess
-    env = dict(os.environ, GPU_MAX_HW_QUEUES=PLAIN_QUEUES)
-    cmd = [sys.executable, os.path.abspath(__file__), "--plain-legs", "1",
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=CONCURRENT_QUEUES)
+    cmd = [sys.executable, os.path.abspath(__file__), "--concurrent-legs", "1",
            "--served-seconds", str(args.served_seconds), "--served-rate", str(args.served_rate),
            "--batched", str(args.batched)]
     r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"plain legs: child exited with {r.returncode}")
+        raise RuntimeError(f"served and batched legs: child exited with {r.returncode}")
     return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def reweight(leg, solo_ms):
+    """Solo-weighted throughput of a served-stream leg with the solo times
+    `solo_ms` (keys "buses/pattern"): both legs are weighted with the solo
+    times measured where one solve runs at a time."""
+    solo = {k.split("/")[1]: v / 1e3 for k, v in solo_ms.items()}
+    done = leg["within_tol_by_pattern"]
+    leg["solo_s_per_s"] = (sum(solo[p] * c for p, c in done.items() if p in solo) / leg["wall_s"]
+                           if leg["wall_s"] > 0 else 0.0)
 
 
 def main():
@@ -253,8 +268,8 @@ def main():
                     help="length of the served and plain request streams (0: skip)")
     ap.add_argument("--served-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served stream (requests/s)")
-    ap.add_argument("--plain-legs", type=int, default=0,
-                    help="(internal) run only the plain legs of the served and batched "
+    ap.add_argument("--concurrent-legs", type=int, default=0,
+                    help="(internal) run only the served and batched legs of the two "
                          "comparisons and print their summaries")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run the plain copy kernel over this many MiB (3 times), the "
@@ -262,18 +277,21 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the KKT solve of one "
-                                   "system does not shard across GPUs (see DESIGN.md)"}))
+                                   "system does not shard across GPUs"}))
         sys.exit(2)
 
-    if args.plain_legs:
+    if args.concurrent_legs:
         legs = {"served": None, "batched": None}
         if args.served_seconds > 0:
             legs["served"] = served_vs_plain(args.served_rate, args.served_seconds,
-                                             legs=("plain",))["plain"]
+                                             legs=("served",))
         if args.batched:
-            legs["batched"] = batched_vs_plain(legs=("plain",))["plain"]
+            legs["batched"] = batched_vs_plain(legs=("batched",))
         print(json.dumps(legs), flush=True)
         return
+    # the served and batched legs first, while no process here holds the GPU
+    conc = (concurrent_legs(args) if args.served_seconds > 0 or args.batched
+            else {"served": None, "batched": None})
 
     import hykkt_amd as H
     from hykkt_amd.generator import ds_spread_schedule, workload
@@ -420,21 +438,25 @@ def main():
     }
     if args.served_seconds > 0 or args.batched:
         s.close()
+    note = (f"served and batched legs first, in a child process at GPU_MAX_HW_QUEUES="
+            f"{CONCURRENT_QUEUES}; the headline sequence and the plain legs after it, in this "
+            f"process at {PLAIN_QUEUES} (HIP's default); one process on the GPU at a time")
     if args.served_seconds > 0:
-        out["served"] = served_vs_plain(args.served_rate, args.served_seconds, legs=("served",))
+        pl = served_vs_plain(args.served_rate, args.served_seconds, legs=("plain",))
+        sv = conc["served"]
+        sv["plain"] = pl["plain"]
+        sv["solo_ms_concurrent_process"] = sv["solo_ms"]
+        sv["solo_ms"] = pl["solo_ms"]
+        for leg in ("served", "plain"):
+            reweight(sv[leg], pl["solo_ms"])
+        sv["queues"] = note
+        out["served"] = sv
     if args.batched:
-        out["batched"] = batched_vs_plain(legs=("batched",))
-    if args.served_seconds > 0 or args.batched:
-        pl = plain_legs(args)
-        note = (f"plain legs in a child process at GPU_MAX_HW_QUEUES={PLAIN_QUEUES} (HIP's "
-                "default), the served and batched legs at 32")
-        if out["served"]:
-            out["served"]["plain"] = pl["served"]
-            out["served"]["queues"] = note
-        if out["batched"]:
-            out["batched"]["plain"] = pl["batched"]
-            out["batched"]["queues"] = note
-        ratios(out["served"], out["batched"])
+        sb = conc["batched"]
+        sb["plain"] = batched_vs_plain(legs=("plain",))["plain"]
+        sb["queues"] = note
+        out["batched"] = sb
+    ratios(out["served"], out["batched"])
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, perm0, args.cpu_sample, gpu_sol)
         e = out["cpu_baseline"]["gpu_err_vs_oracle"]

@@ -420,6 +420,10 @@ def summarize(results, rejected, pending, wall, tol=1e-8, patterns=None, solo_s=
            "max_resid4": max((r.resid4 for r in results), default=None),
            "mean_batch": float(np.mean([r.batch for r in results])) if results else None,
            "latency_after_s": warmup}
+    done = {}
+    for r in ok:
+        done[str(r.pattern)] = done.get(str(r.pattern), 0) + 1
+    out["within_tol_by_pattern"] = done
     if solo_s is not None:
         out["solo_s_per_s"] = (sum(solo_s[r.pattern] for r in ok if r.pattern in solo_s) / wall
                                if wall > 0 else 0.0)

@@ -32,8 +32,10 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st) : st_(st)
 {
   HIPCHK(hipStreamCreate(&st2_));   // default priority, like the solver's stream
-  HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming));
+  // the two streams' events order work on this device only: a device-scope
+  // release, no system-scope cache writeback when they are recorded
+  HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming | hipEventReleaseToDevice));
+  HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming | hipEventReleaseToDevice));
   pl_ = chol_analyze(a, opt);
   if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) {
     tun_.pair_min = std::max(1, std::atoi(pm));
@@ -591,20 +593,23 @@ void CholEngine::factor(double pivot_tol, const dev::Batch& bt)
         rest_pending = false;
         rest_lo = INT_MAX;
       }
+      // the rest waits for this step's next-column update: its event is
+      // recorded by that launch's own completion where there is one
+      const bool rest = L.upd_rest[j].cnt > 0;
+      const bool on_launch = rest && L.upd_next[j].cnt > 0;
       timed(L.upd_next[j].short_rows ? PK_DENSE_NT_SHORT : PK_DENSE_NT, L.upd_next[j], [&] {
         dev::dense_nt(cd_, L.upd_next[j].cnt, t_upd_sn_.p + L.upd_next[j].off,
                       t_upd_tr_.p + L.upd_next[j].off, t_upd_tc_.p + L.upd_next[j].off, umode,
-                      ustep, b, st_, L.upd_next[j].short_rows);
+                      ustep, b, st_, L.upd_next[j].short_rows, on_launch ? ev_main_ : nullptr);
       });
-      if (L.upd_rest[j].cnt > 0) {
-        HIPCHK(hipEventRecord(ev_main_, st_));
+      if (rest) {
+        if (!on_launch) HIPCHK(hipEventRecord(ev_main_, st_));
         HIPCHK(hipStreamWaitEvent(st2_, ev_main_, 0));
         timed_on(st2_, PK_DENSE_NT, L.upd_rest[j], [&] {
           dev::dense_nt(cd_, L.upd_rest[j].cnt, t_upd_sn_.p + L.upd_rest[j].off,
                         t_upd_tr_.p + L.upd_rest[j].off, t_upd_tc_.p + L.upd_rest[j].off, umode,
-                        ustep, b, st2_);
+                        ustep, b, st2_, false, ev_rest_);
         });
-        HIPCHK(hipEventRecord(ev_rest_, st2_));
         rest_pending = true;
         rest_lo = std::min(rest_lo, upd_hi + 1);   // the rest starts after the columns just updated
       }

@@ -237,10 +237,11 @@ void inv_blocks(const CholDev& c, int ntasks, const int* task_sn, const int* tas
 // panel update after block `step`; mode 1: U = -L21 L21^T; mode 2: TRSM of the
 // rows below block `step` with its inverted diagonal block; mode m >= 3: as
 // mode 0 by blocks step .. step+m-2.  short_rows: 32-row x 128-column tiles,
-// tile rows counted in 32-row units (modes 0, 2, 3+).
+// tile rows counted in 32-row units (modes 0, 2, 3+).  done: an event the
+// launch records on completion (nullptr: none).
 void dense_nt(const CholDev& c, int ntasks, const int* task_sn, const int* task_tr,
               const int* task_tc, int mode, int step, const Batch& b, hipStream_t st,
-              bool short_rows = false);
+              bool short_rows = false, hipEvent_t done = nullptr);
 
 // Single-launch supernodal solves over the whole tree.  Work items are
 // 128-row blocks of the fronts, dispatched in a dependency-respecting order;

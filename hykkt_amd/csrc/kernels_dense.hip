@@ -9,6 +9,7 @@
 // with A, B, C column-major.  It feeds the MFMA with B's rows as the A
 // operand and A's rows as the B operand, so each result register holds 16
 // consecutive rows of one C column (coalesced column-major stores).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -513,25 +514,31 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
         double lu[PW];   // unscaled entries (kb+q, kb+k) of rows q > k
 #pragma unroll
         for (int q = k + 1; q < PW; q++) lu[q] = readlane_f64(v[k], q);
+        // 1/sqrt(dkk) straight from the pivot; the pivot test runs beside it
+        // and only selects at the end (a failed pivot is reported and taken
+        // as 1, so the arithmetic stays finite), keeping the scalar unit's
+        // round trip off the column-to-column chain
+        double y = __builtin_amdgcn_rsq(dkk);
+        const double hx = 0.5 * dkk;
+        y = y * fma(-hx, y * y, 1.5);
+        y = y * fma(-hx, y * y, 1.5);
         const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
         badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
-        const double xk = ok ? dkk : 1.0;   // failure reported, arithmetic kept finite
-        double y = __builtin_amdgcn_rsq(xk);
-        const double hx = 0.5 * xk;
-        y = y * fma(-hx, y * y, 1.5);
-        y = y * fma(-hx, y * y, 1.5);
+        y = ok ? y : 1.0;
         // L(r, kb+k) = v_k y; the update L(r, kb+k) L(kb+q, kb+k) is v_k y^2 lu_q
         const double wk = v[k] * (y * y);
-        v[k] = (lane == k) ? xk * y : v[k] * y;
+        v[k] = (lane == k) ? (ok ? dkk * y : 1.0) : v[k] * y;
         myinv = (lane == k) ? y : myinv;
 #pragma unroll
         for (int q = k + 1; q < PW; q++) v[q] -= wk * lu[q];
       }
       if (tid == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
+      // whole rows, also the diagonal block's carried entries above the
+      // diagonal: that half of a diagonal tile is scratch until the inverse
+      // is written there (one store mask, no per-entry branches)
       if (mine) {
 #pragma unroll
-        for (int q = 0; q < PW; q++)
-          if (!dg || q <= lane) P[r][kb + q] = v[q];
+        for (int q = 0; q < PW; q++) P[r][kb + q] = v[q];
       }
       if (wave == 0 && dg) rd[kb + lane] = myinv;
       __syncthreads();
@@ -842,14 +849,18 @@ __global__ void __launch_bounds__(256) k_trsm(CholDev c, const int* __restrict__
 
 size_t dense_nt_smem() { return 2 * KC * LDA_S * sizeof(double); }
 void dense_nt(const CholDev& c, int ntasks, const int* tsn, const int* ttr, const int* ttc,
-              int mode, int step, const Batch& b, hipStream_t st, bool short_rows)
+              int mode, int step, const Batch& b, hipStream_t st, bool short_rows, hipEvent_t done)
 {
   if (ntasks <= 0 || b.count <= 0) return;
   const dim3 grid(ntasks, b.count);
+  // `done` rides on the launch itself (its completion signal), so no marker
+  // of its own sits between this kernel and the next one of the stream
   if (short_rows)
-    k_dense_nt<32><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
+    hipExtLaunchKernelGGL(k_dense_nt<32>, grid, dim3(256), dense_nt_smem(), st, nullptr, done, 0,
+                          c, tsn, ttr, ttc, mode, step, b);
   else
-    k_dense_nt<128><<<grid, 256, dense_nt_smem(), st>>>(c, tsn, ttr, ttc, mode, step, b);
+    hipExtLaunchKernelGGL(k_dense_nt<128>, grid, dim3(256), dense_nt_smem(), st, nullptr, done, 0,
+                          c, tsn, ttr, ttc, mode, step, b);
   HIPCHK(hipGetLastError());
 }
 

@@ -397,6 +397,6 @@ def test_delta_hook_gpu():
     s.execute()
     st = s.stats()
     assert st["refine_steps"] >= 1 and st["resid4"] <= 1e-9
-    s.options(delta=0.0)
+    s.options(delta=0.0, refine_tol=9e-9)   # the first execute's options again
     assert s.execute() == 0
     assert np.array_equal(np.concatenate(s.solution()), x_plain)

@@ -440,6 +440,50 @@ __device__ __forceinline__ void inverse_full(double (*P)[BK + 1], double* W, int
   }
 }
 
+// P[r][c] -= sum_k P[r][kb + k] P[c][kb + k] (k < 16, the panel at column
+// kb) over the 16 x 16 tiles e < nt whose top-left corners tile(e, r0, c0)
+// gives, tile e to wave w0 + e % nw (the calling waves are w0 .. w0+nw-1),
+// two tiles at a time: all operand reads first, the two MFMA chains
+// interleaved.  Whole tiles: the upper half of a diagonal tile is scratch
+// until the inverse is written.
+template <int BK, class TILE>
+__device__ __forceinline__ void panel_tiles(double (*P)[BK + 1], int kb, int nt, int w0, int nw,
+                                            int wave, int lane, TILE tile)
+{
+  const int i = lane & 15, kq = lane >> 4;
+  for (int e = wave - w0; e < nt; e += 2 * nw) {
+    int r0[2], c0[2];
+    const bool two = e + nw < nt;
+    tile(e, r0[0], c0[0]);
+    tile(two ? e + nw : e, r0[1], c0[1]);
+    d4 acc[2];
+    double fa[2][4], fb[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[h][v] = P[r0[h] + 4 * v + kq][c0[h] + i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        fa[h][k] = -P[r0[h] + i][kb + 4 * k + kq];
+        fb[h][k] = P[c0[h] + i][kb + 4 * k + kq];
+      }
+    }
+    // C(r, c) -= sum_k A(r, k) B(c, k): result element v is row 4v + lane/16
+    // of the tile, column lane % 16
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+        acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[h][k], fb[h][k], acc[h], 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; v++) P[r0[0] + 4 * v + kq][c0[0] + i] = acc[0][v];
+    if (two) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) P[r0[1] + 4 * v + kq][c0[1] + i] = acc[1][v];
+    }
+  }
+}
+
 // Diagonal block `step` of supernode s, by one workgroup of 256 threads with
 // smem >= potrf_smem<BK>().  BK: padded block edge (32, 64 or 128) >= b.
 // FULL: the whole inverse; otherwise only the inverses of its 16x16 diagonal
@@ -485,112 +529,94 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
   __syncthreads();
   const double tol2 = c.pivot_tol * c.pivot_tol;
 
-  // ---- blocked Cholesky ----
-  // Each wave factors the panel's 16 x 16 diagonal block in lanes 0..15 (lane
-  // l holds row kb + l; every wave the same) and 48 of the rows below in
-  // lanes 16..63, so the whole panel is factored in one pass: at step k the
-  // pivot row's entries reach every lane of the wave through v_readlane
-  // (read before the pivot's 1/sqrt is known, scaled after), column k is
-  // scaled and the later columns take its rank-1 update.  Entries above the
-  // diagonal of the 16 x 16 block are carried along, never stored, so no
-  // lane-dependent branches.
+  // ---- blocked Cholesky, one 16-column panel at a time ----
+  // Each factoring wave takes the panel's 16 x 16 diagonal block in lanes
+  // 0..15 (lane l holds row kb + l; every such wave the same) and 48 of the
+  // rows below in lanes 16..63, so the whole panel is factored in one pass:
+  // at step k the pivot row's entries reach every lane of the wave through
+  // v_readlane (read before the pivot's 1/sqrt is known, scaled after),
+  // column k is scaled and the later columns take its rank-1 update.  Entries
+  // above the diagonal of the 16 x 16 block are carried along, never stored,
+  // so no lane-dependent branches.  Only as many waves factor as the panel's
+  // rows need (three for the first panel, then two, then one); the others
+  // meanwhile apply the previous panel's update to the tile columns after
+  // this one.  After each panel, all waves update the next tile column by it.
   {
     const bool dg = lane < PW;
     const int prow_off = dg ? lane : PW + 48 * wave + (lane - PW);   // row - kb
 #pragma unroll 1
     for (int p = 0; p < NPN; p++) {
       const int kb = p * PW;
-      const int r = kb + prow_off;
-      const bool mine = r < BK && (dg ? wave == 0 : true);   // this lane stores row r
-      const int rr = min(r, BK - 1);
-      double v[PW];
+      const int nf = max(1, (BK - kb - PW + 47) / 48);   // factoring waves
+      if (wave < nf) {
+        const int r = kb + prow_off;
+        const bool mine = r < BK && (dg ? wave == 0 : true);   // this lane stores row r
+        const int rr = min(r, BK - 1);
+        double v[PW];
 #pragma unroll
-      for (int q = 0; q < PW; q++) v[q] = P[rr][kb + q];
-      int badk = -1;
-      double myinv = 1.0;   // 1 / L(kb + lane, kb + lane), taken at step k == lane
+        for (int q = 0; q < PW; q++) v[q] = P[rr][kb + q];
+        int badk = -1;
+        double myinv = 1.0;   // 1 / L(kb + lane, kb + lane), taken at step k == lane
 #pragma unroll
-      for (int k = 0; k < PW; k++) {
-        const double dkk = readlane_f64(v[k], k);
-        double lu[PW];   // unscaled entries (kb+q, kb+k) of rows q > k
+        for (int k = 0; k < PW; k++) {
+          const double dkk = readlane_f64(v[k], k);
+          double lu[PW];   // unscaled entries (kb+q, kb+k) of rows q > k
 #pragma unroll
-        for (int q = k + 1; q < PW; q++) lu[q] = readlane_f64(v[k], q);
-        // 1/sqrt(dkk) straight from the pivot; the pivot test runs beside it
-        // and only selects at the end (a failed pivot is reported and taken
-        // as 1, so the arithmetic stays finite), keeping the scalar unit's
-        // round trip off the column-to-column chain
-        double y = __builtin_amdgcn_rsq(dkk);
-        const double hx = 0.5 * dkk;
-        y = y * fma(-hx, y * y, 1.5);
-        y = y * fma(-hx, y * y, 1.5);
-        const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
-        badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
-        y = ok ? y : 1.0;
-        // L(r, kb+k) = v_k y; the update L(r, kb+k) L(kb+q, kb+k) is v_k y^2 lu_q
-        const double wk = v[k] * (y * y);
-        v[k] = (lane == k) ? (ok ? dkk * y : 1.0) : v[k] * y;
-        myinv = (lane == k) ? y : myinv;
+          for (int q = k + 1; q < PW; q++) lu[q] = readlane_f64(v[k], q);
+          // 1/sqrt(dkk) straight from the pivot; the pivot test runs beside
+          // it and only selects at the end (a failed pivot is reported and
+          // taken as 1, so the arithmetic stays finite), keeping the scalar
+          // unit's round trip off the column-to-column chain
+          double y = __builtin_amdgcn_rsq(dkk);
+          const double hx = 0.5 * dkk;
+          y = y * fma(-hx, y * y, 1.5);
+          y = y * fma(-hx, y * y, 1.5);
+          const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
+          badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
+          y = ok ? y : 1.0;
+          // L(r, kb+k) = v_k y; the update L(r, kb+k) L(kb+q, kb+k) is v_k y^2 lu_q
+          const double wk = v[k] * (y * y);
+          v[k] = (lane == k) ? (ok ? dkk * y : 1.0) : v[k] * y;
+          myinv = (lane == k) ? y : myinv;
+          // formed here, so the step's masks end with the step (the compiler
+          // would otherwise defer these selects to the stores, holding every
+          // step's masks in scalar registers)
+          asm volatile("" : "+v"(v[k]), "+v"(myinv));
 #pragma unroll
-        for (int q = k + 1; q < PW; q++) v[q] -= wk * lu[q];
-      }
-      if (tid == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
-      // whole rows, also the diagonal block's carried entries above the
-      // diagonal: that half of a diagonal tile is scratch until the inverse
-      // is written there (one store mask, no per-entry branches)
-      if (mine) {
-#pragma unroll
-        for (int q = 0; q < PW; q++) P[r][kb + q] = v[q];
-      }
-      if (wave == 0 && dg) rd[kb + lane] = myinv;
-      __syncthreads();
-    const int rbeg = kb + PW;
-    if (rbeg < BK) {
-      // trailing update P[r][cc] -= sum_k P[r][k] P[cc][k] over the panel,
-      // r >= cc >= rbeg, in 16x16 tiles (lower tiles only; the upper half of
-      // a diagonal tile is scratch until the inverse is written)
-      // Tile e goes to wave e % 4; a wave takes two tiles at a time (all
-      // operand reads first, the two MFMA chains interleaved).
-      const int nrt = (BK - rbeg) / PW, nt = nrt * (nrt + 1) / 2;
-      const int i = lane & 15, kq = lane >> 4;
-      for (int e = wave; e < nt; e += 8) {
-        int r0[2], c0[2];
-        bool two = e + 4 < nt;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const int et = min(e + 4 * h, nt - 1);
-          int ti = 0;
-          while ((ti + 1) * (ti + 2) / 2 <= et) ti++;
-          r0[h] = rbeg + PW * ti;
-          c0[h] = rbeg + PW * (et - ti * (ti + 1) / 2);
+          for (int q = k + 1; q < PW; q++) v[q] -= wk * lu[q];
         }
-        d4 acc[2];
-        double fa[2][4], fb[2][4];
+        if (tid == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
+        // whole rows, also the diagonal block's carried entries above the
+        // diagonal: that half of a diagonal tile is scratch until the inverse
+        // is written there (one store mask, no per-entry branches)
+        if (mine) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc[h][v] = P[r0[h] + 4 * v + kq][c0[h] + i];
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            fa[h][k] = -P[r0[h] + i][kb + 4 * k + kq];
-            fb[h][k] = P[c0[h] + i][kb + 4 * k + kq];
-          }
+          for (int q = 0; q < PW; q++) P[r][kb + q] = v[q];
         }
-        // C(r, c) -= sum_k A(r, k) B(c, k): result element v is row 4v + lane/16
-        // of the tile, column lane % 16
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-#pragma unroll
-          for (int h = 0; h < 2; h++)
-            acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[h][k], fb[h][k], acc[h], 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; v++) P[r0[0] + 4 * v + kq][c0[0] + i] = acc[0][v];
-        if (two) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) P[r0[1] + 4 * v + kq][c0[1] + i] = acc[1][v];
-        }
+        if (wave == 0 && dg) rd[kb + lane] = myinv;
+      } else if (p > 0) {
+        // the previous panel's update of the tile columns after this panel:
+        // the lower tiles of the block from row and column rb on
+        const int rb = kb + PW, m = (BK - rb) / PW;
+        panel_tiles<BK>(P, kb - PW, m * (m + 1) / 2, nf, 4 - nf, wave, lane,
+                        [&](int e, int& r0, int& c0) {
+                          int ti = 0;
+                          while ((ti + 1) * (ti + 2) / 2 <= e) ti++;
+                          r0 = rb + PW * ti;
+                          c0 = rb + PW * (e - ti * (ti + 1) / 2);
+                        });
       }
       __syncthreads();
+      if (kb + PW < BK) {
+        // this panel's update of the next tile column
+        const int cb = kb + PW;
+        panel_tiles<BK>(P, kb, (BK - cb) / PW, 0, 4, wave, lane, [&](int e, int& r0, int& c0) {
+          r0 = cb + PW * e;
+          c0 = cb;
+        });
+        __syncthreads();
+      }
     }
-  }
   }
 
   // L_kk back to the factor (the inverse below keeps P's lower part), the

@@ -440,15 +440,15 @@ __device__ __forceinline__ void inverse_full(double (*P)[BK + 1], double* W, int
   }
 }
 
-// P[r][c] -= sum_k P[r][kb + k] P[c][kb + k] (k < 16, the panel at column
-// kb) over the 16 x 16 tiles e < nt whose top-left corners tile(e, r0, c0)
+// P[r][c] -= sum_k P[r][k] P[c][k] over the np 16-column panels from column
+// kb0 on, for the 16 x 16 tiles e < nt whose top-left corners tile(e, r0, c0)
 // gives, tile e to wave w0 + e % nw (the calling waves are w0 .. w0+nw-1),
-// two tiles at a time: all operand reads first, the two MFMA chains
-// interleaved.  Whole tiles: the upper half of a diagonal tile is scratch
-// until the inverse is written.
+// two tiles at a time: each panel's operand reads first, the two MFMA
+// chains interleaved.  Whole tiles: the upper half of a diagonal tile is
+// scratch until the inverse is written.
 template <int BK, class TILE>
-__device__ __forceinline__ void panel_tiles(double (*P)[BK + 1], int kb, int nt, int w0, int nw,
-                                            int wave, int lane, TILE tile)
+__device__ __forceinline__ void panel_tiles(double (*P)[BK + 1], int kb0, int np, int nt, int w0,
+                                            int nw, int wave, int lane, TILE tile)
 {
   const int i = lane & 15, kq = lane >> 4;
   for (int e = wave - w0; e < nt; e += 2 * nw) {
@@ -457,24 +457,28 @@ __device__ __forceinline__ void panel_tiles(double (*P)[BK + 1], int kb, int nt,
     tile(e, r0[0], c0[0]);
     tile(two ? e + nw : e, r0[1], c0[1]);
     d4 acc[2];
-    double fa[2][4], fb[2][4];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < 2; h++)
 #pragma unroll
       for (int v = 0; v < 4; v++) acc[h][v] = P[r0[h] + 4 * v + kq][c0[h] + i];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        fa[h][k] = -P[r0[h] + i][kb + 4 * k + kq];
-        fb[h][k] = P[c0[h] + i][kb + 4 * k + kq];
-      }
-    }
-    // C(r, c) -= sum_k A(r, k) B(c, k): result element v is row 4v + lane/16
-    // of the tile, column lane % 16
-#pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int q = 0; q < np; q++) {
+      const int kb = kb0 + PW * q;
+      double fa[2][4], fb[2][4];
 #pragma unroll
       for (int h = 0; h < 2; h++)
-        acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[h][k], fb[h][k], acc[h], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          fa[h][k] = -P[r0[h] + i][kb + 4 * k + kq];
+          fb[h][k] = P[c0[h] + i][kb + 4 * k + kq];
+        }
+      // C(r, c) -= sum_k A(r, k) B(c, k): result element v is row 4v + lane/16
+      // of the tile, column lane % 16
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[h][k], fb[h][k], acc[h], 0, 0, 0);
+    }
 #pragma unroll
     for (int v = 0; v < 4; v++) P[r0[0] + 4 * v + kq][c0[0] + i] = acc[0][v];
     if (two) {
@@ -527,7 +531,6 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
   }
   if (tid == 0) bad = INT_MAX;
   __syncthreads();
-  const double tol2 = c.pivot_tol * c.pivot_tol;
 
   // ---- blocked Cholesky, one 16-column panel at a time ----
   // Each factoring wave takes the panel's 16 x 16 diagonal block in lanes
@@ -539,8 +542,9 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
   // above the diagonal of the 16 x 16 block are carried along, never stored,
   // so no lane-dependent branches.  Only as many waves factor as the panel's
   // rows need (three for the first panel, then two, then one); the others
-  // meanwhile apply the previous panel's update to the tile columns after
-  // this one.  After each panel, all waves update the next tile column by it.
+  // meanwhile bring the next tile column up to date with all earlier panels
+  // (left-looking, so this work stays small and even), and after the panel
+  // all waves apply the panel itself to that column.
   {
     const bool dg = lane < PW;
     const int prow_off = dg ? lane : PW + 48 * wave + (lane - PW);   // row - kb
@@ -555,7 +559,6 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
         double v[PW];
 #pragma unroll
         for (int q = 0; q < PW; q++) v[q] = P[rr][kb + q];
-        int badk = -1;
         double myinv = 1.0;   // 1 / L(kb + lane, kb + lane), taken at step k == lane
 #pragma unroll
         for (int k = 0; k < PW; k++) {
@@ -563,20 +566,15 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
           double lu[PW];   // unscaled entries (kb+q, kb+k) of rows q > k
 #pragma unroll
           for (int q = k + 1; q < PW; q++) lu[q] = readlane_f64(v[k], q);
-          // 1/sqrt(dkk) straight from the pivot; the pivot test runs beside
-          // it and only selects at the end (a failed pivot is reported and
-          // taken as 1, so the arithmetic stays finite), keeping the scalar
-          // unit's round trip off the column-to-column chain
+          // 1/sqrt(dkk), v_rsq_f64 and two Newton steps; the pivot test
+          // comes after the panel, off the column-to-column chain
           double y = __builtin_amdgcn_rsq(dkk);
           const double hx = 0.5 * dkk;
           y = y * fma(-hx, y * y, 1.5);
           y = y * fma(-hx, y * y, 1.5);
-          const bool ok = (dkk > 0.0) && (dkk > tol2);   // sqrt(dkk) > pivot_tol
-          badk = (badk < 0 && !ok && kb + k < b) ? k : badk;
-          y = ok ? y : 1.0;
           // L(r, kb+k) = v_k y; the update L(r, kb+k) L(kb+q, kb+k) is v_k y^2 lu_q
           const double wk = v[k] * (y * y);
-          v[k] = (lane == k) ? (ok ? dkk * y : 1.0) : v[k] * y;
+          v[k] = (lane == k) ? dkk * y : v[k] * y;
           myinv = (lane == k) ? y : myinv;
           // formed here, so the step's masks end with the step (the compiler
           // would otherwise defer these selects to the stores, holding every
@@ -585,7 +583,6 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
 #pragma unroll
           for (int q = k + 1; q < PW; q++) v[q] -= wk * lu[q];
         }
-        if (tid == 0 && badk >= 0) bad = si.a + k0 + kb + badk;
         // whole rows, also the diagonal block's carried entries above the
         // diagonal: that half of a diagonal tile is scratch until the inverse
         // is written there (one store mask, no per-entry branches)
@@ -593,24 +590,32 @@ __device__ __forceinline__ void potrf_block(const CholDev& c, double* L, double*
 #pragma unroll
           for (int q = 0; q < PW; q++) P[r][kb + q] = v[q];
         }
-        if (wave == 0 && dg) rd[kb + lane] = myinv;
-      } else if (p > 0) {
-        // the previous panel's update of the tile columns after this panel:
-        // the lower tiles of the block from row and column rb on
-        const int rb = kb + PW, m = (BK - rb) / PW;
-        panel_tiles<BK>(P, kb - PW, m * (m + 1) / 2, nf, 4 - nf, wave, lane,
+        if (wave == 0) {
+          // the pivot test, sqrt(pivot) > pivot_tol, on L(kb+l, kb+l) as lane l
+          // just stored it: a failed pivot leaves a small value or NaN there
+          // (NaN then runs through the rest of this system's factor; the
+          // system is reported and kept out of the solve).  The first
+          // failing column of the first failing panel is kept.
+          const double lkk = P[kb + (lane & 15)][kb + (lane & 15)];
+          const uint64_t f = __builtin_amdgcn_ballot_w64(dg && kb + lane < b && !(lkk > c.pivot_tol));
+          if (f != 0 && tid == 0 && bad == INT_MAX) bad = si.a + k0 + kb + __builtin_ctzll(f);
+          if (dg) rd[kb + lane] = myinv;
+        }
+      } else if (p > 0 && kb + PW < BK) {
+        // the next tile column (its tiles from the diagonal down) by the
+        // panels before this one
+        const int cb = kb + PW;
+        panel_tiles<BK>(P, 0, p, (BK - cb) / PW, nf, 4 - nf, wave, lane,
                         [&](int e, int& r0, int& c0) {
-                          int ti = 0;
-                          while ((ti + 1) * (ti + 2) / 2 <= e) ti++;
-                          r0 = rb + PW * ti;
-                          c0 = rb + PW * (e - ti * (ti + 1) / 2);
+                          r0 = cb + PW * e;
+                          c0 = cb;
                         });
       }
       __syncthreads();
       if (kb + PW < BK) {
-        // this panel's update of the next tile column
+        // and by this panel
         const int cb = kb + PW;
-        panel_tiles<BK>(P, kb, (BK - cb) / PW, 0, 4, wave, lane, [&](int e, int& r0, int& c0) {
+        panel_tiles<BK>(P, kb, 1, (BK - cb) / PW, 0, 4, wave, lane, [&](int e, int& r0, int& c0) {
           r0 = cb + PW * e;
           c0 = cb;
         });

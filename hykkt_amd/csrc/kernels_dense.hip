@@ -198,6 +198,9 @@ __global__ void __launch_bounds__(256, 2) k_dense_nt(CholDev c, const int* __res
                                                   const int* __restrict__ ttc, int mode, int step,
                                                   Batch bt)
 {
+  // the 32-row launches are the factorization's critical chain: their waves
+  // take issue priority over the trailing updates they share SIMDs with
+  if constexpr (TM == 32) __builtin_amdgcn_s_setprio(3);
   extern __shared__ double smem[];
   const int sy = batch_sys(bt);
   if (sy < 0) return;
@@ -676,6 +679,7 @@ template <int BK>
 __global__ void __launch_bounds__(256) k_potrf_inv(CholDev c, const int* __restrict__ sns,
                                                    int step, Batch bt)
 {
+  __builtin_amdgcn_s_setprio(3);   // the critical chain: issue priority (see k_dense_nt)
   extern __shared__ double smem[];
   const int sy = batch_sys(bt);
   if (sy < 0) return;
@@ -759,6 +763,7 @@ template <int TM>
 __global__ void __launch_bounds__(256) k_trsm(CholDev c, const int* __restrict__ tsn,
                                               const int* __restrict__ ttr, int step, Batch bt)
 {
+  __builtin_amdgcn_s_setprio(3);   // the critical chain: issue priority (see k_dense_nt)
   constexpr int NG = TM / 16, NGW = (NG + 3) / 4;   // groups, groups per wave
   constexpr int NPN = BT / PW;                       // panels
   extern __shared__ double smem[];

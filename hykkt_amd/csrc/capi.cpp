@@ -830,7 +830,7 @@ int hykkt_schur_cg(hykkt_chol c, int m, int n, const int* j_rp, const int* j_ci,
     for (DevBuf<double>* q : {&r, &w, &p, &s}) q->alloc(std::max(1, m));
     sc.alloc(8);
     part.alloc(hykkt::schur_partials(m, static_cast<int>(jb.size()) - 1));
-    gath.alloc(2);
+    gath.alloc(4);
     flg.alloc(3);
     hykkt::SchurOps o;
     o.m = m;

@@ -87,10 +87,7 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
   // host copies: pinned when the caller gave memory for them
   std::vector<double> own(o.pinned ? 0 : 4 * static_cast<size_t>(cnt));
   double* hostmem = o.pinned ? o.pinned : own.data();
-  int* lv = reinterpret_cast<int*>(hostmem);            // cnt ints
-  int* it = lv + cnt;                                    // cnt ints
-  int* stt = it + cnt;                                   // cnt ints
-  double* sc = hostmem + 2 * static_cast<size_t>(cnt);   // 2 cnt doubles (after 4 cnt ints)
+  int* lv = reinterpret_cast<int*>(hostmem);   // cnt ints; the results reuse the memory
   int next_check = std::max(1, chunk);
   for (int i = 0; i < itmax; i++) {
     dev::cg_update_dot(m, o.sc, o.p, o.s, o.x, o.r, o.w, o.part, live, o.st);
@@ -107,18 +104,23 @@ std::vector<CgResult> schur_cg_run(SchurOps& o, const double* b, double tol, int
     }
   }
   // per system: iterations, status and ||r|| at exit (gam1, or gam for a
-  // system that stopped at the start)
-  dev::collect(o.gath, o.sc, 2, dev::Batch{cnt, bt.vs, nullptr}, o.st);
-  HIPCHK(hipMemcpyAsync(it, o.iters, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
-  HIPCHK(hipMemcpyAsync(stt, o.status, sizeof(int) * cnt, hipMemcpyDeviceToHost, o.st));
-  HIPCHK(hipMemcpyAsync(sc, o.gath, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost, o.st));
+  // system that stopped at the start), packed on the device, one copy
+  dev::cg_pack(o.gath, o.sc, o.iters, o.status, dev::Batch{cnt, bt.vs, nullptr}, o.st);
+  HIPCHK(hipMemcpyAsync(hostmem, o.gath, sizeof(double) * 4 * cnt, hipMemcpyDeviceToHost, o.st));
+  if (o.defer && o.pinned) return res;
   HIPCHK(hipStreamSynchronize(o.st));
-  for (int k = 0; k < cnt; k++) {
-    res[k].iters = it[k];
-    res[k].status = stt[k];
-    res[k].final_norm = std::sqrt(it[k] == 0 ? sc[2 * k] : sc[2 * k + 1]);
-  }
+  cg_unpack(hostmem, res);
   return res;
+}
+
+void cg_unpack(const double* pinned, std::vector<CgResult>& res)
+{
+  for (size_t k = 0; k < res.size(); k++) {
+    const double* h = pinned + 4 * k;
+    res[k].iters = static_cast<int>(h[2]);
+    res[k].status = static_cast<int>(h[3]);
+    res[k].final_norm = std::sqrt(res[k].iters == 0 ? h[0] : h[1]);
+  }
 }
 
 }  // namespace hykkt

@@ -238,12 +238,16 @@ struct SchurOps {
   // (q the same image of the direction p), so the recovery needs no solve
   double *tq = nullptr, *tz = nullptr;
   int *live = nullptr, *iters = nullptr, *status = nullptr;   // per system
-  double* gath = nullptr;   // 2 doubles per system: scalars gathered for the readback
+  double* gath = nullptr;   // 4 doubles per system: the results packed for the readback
   Profiler* prof = nullptr;  // optional launch timing of the SpMVs
   int nnz = 0;               // entries of Jc
   // optional pinned host memory of at least 4 doubles per system, for the
   // live-flag checks and the results (pageable vectors otherwise)
   double* pinned = nullptr;
+  // with pinned memory: leave the results' copy in flight and return them
+  // unfilled; cg_unpack(pinned, ...) fills them after the caller's next
+  // synchronization of the stream (one host round trip less per pass)
+  bool defer = false;
 };
 // Partials a SchurOps needs per system.
 int schur_partials(int m, int jh_nblk);
@@ -264,5 +268,7 @@ struct CgResult {
 std::vector<CgResult> schur_cg_run(SchurOps& ops, const double* b, double tol, int itmax,
                                    const dev::Batch& bt, bool x0_zero, int chunk,
                                    int chunk_next = 1);
+// The results of a deferred schur_cg_run from its pinned memory.
+void cg_unpack(const double* pinned, std::vector<CgResult>& res);
 
 }  // namespace hykkt

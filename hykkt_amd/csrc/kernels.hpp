@@ -124,6 +124,24 @@ int reduce_blocks(int64_t n);
 // `part` needs reduce_blocks(n) doubles per system.
 void dot(int n, const double* a, const double* bv, double* part, double* out, const Batch& b,
          hipStream_t st);
+// Up to DOTS_MAX such sums in two launches, out[k] per system for sum k
+// (same bits as dot() each); part needs the sum of their reduce_blocks(n[k])
+// doubles per system.  off is filled in by dots().
+constexpr int DOTS_MAX = 8;
+struct DotSet {
+  const double* a[DOTS_MAX];
+  const double* b[DOTS_MAX];
+  int n[DOTS_MAX];
+  int off[DOTS_MAX + 1];
+  int count = 0;
+};
+void dots(DotSet d, double* part, double* out, const Batch& b, hipStream_t st);
+// Per system k (b.count systems, stride b.vs between sc slots): sc[0], sc[1],
+// iters[k], status[k] as out[4k .. 4k+3].
+void cg_pack(double* out, const double* sc, const int* iters, const int* status, const Batch& b,
+             hipStream_t st);
+// out[0] = *w, then a[0..count) and bv[0..count) where not null.
+void collect_ints(int* out, const int* w, const int* a, const int* bv, int count, hipStream_t st);
 // The first stage of dot(): reduce_blocks(n) partials per system in part.
 void dot_partial(int n, const double* a, const double* bv, double* part, const Batch& b,
                  hipStream_t st);

@@ -129,6 +129,36 @@ __global__ void k_collect(double* __restrict__ out, const double* __restrict__ b
   out[t] = base[k * b.vs + j];
 }
 
+// Per system k: the CG's ||r||^2 at the start and at exit (sc slots 0, 1),
+// its iterations and status, as four doubles for one readback.
+__global__ void k_cg_pack(double* __restrict__ out, const double* __restrict__ sc,
+                          const int* __restrict__ iters, const int* __restrict__ status, Batch b)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.count) return;
+  const double* c = sc + static_cast<int64_t>(k) * b.vs;
+  out[4 * k + 0] = c[0];
+  out[4 * k + 1] = c[1];
+  out[4 * k + 2] = iters[k];
+  out[4 * k + 3] = status[k];
+}
+
+// out[0] = *w, then a[0..count) and b[0..count) where given.
+__global__ void k_collect_ints(int* __restrict__ out, const int* __restrict__ w,
+                               const int* __restrict__ a, const int* __restrict__ bv, int count)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) out[0] = *w;
+  if (t < count) {
+    int at = 1;
+    if (a) {
+      out[at + t] = a[t];
+      at += count;
+    }
+    if (bv) out[at + t] = bv[t];
+  }
+}
+
 __global__ void k_gather(double* __restrict__ d, const double* __restrict__ src,
                          const int* __restrict__ map, int64_t n, Batch b)
 {
@@ -451,6 +481,44 @@ __global__ void k_dot_partial(int n, const double* __restrict__ a, const double*
     t = bv ? fma(a[i], bv[i], t) : t + a[i];
   const double tot = block_sum(t);
   if (threadIdx.x == 0) at(part, s, b)[blockIdx.x] = tot;
+}
+
+// Several sums in one launch: block j of sum k (off[k] <= j < off[k+1])
+// takes the block j - off[k] of the g = off[k+1] - off[k] blocks that
+// k_dot_partial would launch for it, in the same order, so the partials are
+// the same bits.
+__global__ void k_dots_partial(DotSet d, double* __restrict__ part, Batch b)
+{
+  const int s = sys_of(b);
+  if (s < 0) return;
+  int k = 0;
+  while (k + 1 < d.count && d.off[k + 1] <= static_cast<int>(blockIdx.x)) k++;
+  const int jb = blockIdx.x - d.off[k], g = d.off[k + 1] - d.off[k];
+  const double* a = at(d.a[k], s, b);
+  const double* bv = d.b[k] ? at(d.b[k], s, b) : nullptr;
+  const int n = d.n[k];
+  double t = 0.0;
+  for (int i = jb * TPB + threadIdx.x; i < n; i += g * TPB)
+    t = bv ? fma(a[i], bv[i], t) : t + a[i];
+  const double tot = block_sum(t);
+  if (threadIdx.x == 0) at(part, s, b)[blockIdx.x] = tot;
+}
+
+// one workgroup per system: each sum's partials added as k_dot_final adds them
+__global__ void k_dots_final(DotSet d, const double* __restrict__ part, double* __restrict__ out,
+                             Batch b)
+{
+  const int s = sys_of(b);
+  if (s < 0) return;
+  part = at(part, s, b);
+  out = at(out, s, b);
+  for (int k = 0; k < d.count; k++) {
+    const int np = d.off[k + 1] - d.off[k];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < np; i += TPB) t += part[d.off[k] + i];
+    const double tot = block_sum(t);
+    if (threadIdx.x == 0) out[k] = tot;
+  }
 }
 
 // one workgroup per system: the sum of its partials, then the CG step's
@@ -889,6 +957,31 @@ void dot(int n, const double* a, const double* bv, double* part, double* out, co
   // out is a per-system slot: sc = out, slot 0
   k_dot_final<<<dim3(1, b.count), TPB, 0, st>>>(g, part, out, 0, 0, 0, 0, 0.0, 0,
                                                 CgFlags{nullptr, nullptr, nullptr}, b);
+  LAUNCH_CHECK();
+}
+
+void dots(DotSet d, double* part, double* out, const Batch& b, hipStream_t st)
+{
+  if (b.count <= 0 || d.count <= 0) return;
+  d.off[0] = 0;
+  for (int k = 0; k < d.count; k++) d.off[k + 1] = d.off[k] + reduce_blocks(d.n[k]);
+  k_dots_partial<<<dim3(d.off[d.count], b.count), TPB, 0, st>>>(d, part, b);
+  LAUNCH_CHECK();
+  k_dots_final<<<dim3(1, b.count), TPB, 0, st>>>(d, part, out, b);
+  LAUNCH_CHECK();
+}
+
+void cg_pack(double* out, const double* sc, const int* iters, const int* status, const Batch& b,
+             hipStream_t st)
+{
+  if (b.count <= 0) return;
+  k_cg_pack<<<nblocks(b.count), TPB, 0, st>>>(out, sc, iters, status, b);
+  LAUNCH_CHECK();
+}
+
+void collect_ints(int* out, const int* w, const int* a, const int* bv, int count, hipStream_t st)
+{
+  k_collect_ints<<<nblocks(std::max(count, 1)), TPB, 0, st>>>(out, w, a, bv, count);
   LAUNCH_CHECK();
 }
 

@@ -417,7 +417,7 @@ void Solver::analyze()
   z.cs = slot(m);
   z.schur = slot(m);
   z.sc = slot(8);
-  z.part = slot(std::max(256, schur_partials(m, S->jch_nblk)));
+  z.part = slot(std::max(8 * 256, schur_partials(m, S->jch_nblk)));   // 8 residual sums
   z.dx = slot(n);
   z.dsv = slot(d);
   z.dyc = slot(m);
@@ -451,7 +451,7 @@ void Solver::analyze()
   S->cap = cap_;
   S->arena.alloc(static_cast<size_t>(z.stride) * cap_);
   S->flags.alloc(static_cast<size_t>(FL_COUNT) * cap_);
-  S->gath.alloc(static_cast<size_t>(RES_SLOTS) * cap_);
+  S->gath.alloc(static_cast<size_t>(RES_SLOTS) * cap_ + cap_ + 1);   // + read_res's ints
   S->gam_dev.alloc(static_cast<size_t>(cap_));
   S->tab.alloc(static_cast<size_t>(IN_COUNT + 1) * cap_);
   double* base = S->arena.p;
@@ -740,18 +740,19 @@ std::vector<int> Solver::run(int count, bool from_setters)
                    P(z.dyd), b, st_);
   const dev::Batch bok = dev::masked(b, S.flag(FL_OK));
   // CG start y_host, or zero when none is bound (then no start-residual solve)
-  const std::vector<CgResult> cg = solve_phase(Pass{z.rxp, z.schur, z.cx}, z.rx_til, z.ry_s,
-                                               z.in[IN_RS], z.in[IN_RYD], any_y0, z.dx, z.dsv,
-                                               z.dyc, z.dyd, bok, ev_[2], last_iters_);
-  int it_max = 0;
-  for (const CgResult& r : cg) it_max = std::max(it_max, r.iters);
-  last_iters_ = it_max;
+  std::vector<CgResult> cg = solve_phase(Pass{z.rxp, z.schur, z.cx}, z.rx_til, z.ry_s,
+                                         z.in[IN_RS], z.in[IN_RYD], any_y0, z.dx, z.dsv, z.dyc,
+                                         z.dyd, bok, ev_[2], last_iters_);
 
   // ---- check_error (HykktSolver.cpp:846-930) + rows 2 and 4 ----
   residual(z.dx, z.dsv, z.dyc, z.dyd, true, b);
   HIPCHK(hipEventRecord(ev_[3], st_));
   std::vector<int> zrow(count), keep(count), act(count), piv(count);
   std::vector<double> hr = read_res(count, &zrow, &piv);
+  cg_unpack(S.cg_host.p, cg);   // its copy finished with read_res's synchronization
+  int it_max = 0;
+  for (const CgResult& r : cg) it_max = std::max(it_max, r.iters);
+  last_iters_ = it_max;
   std::vector<int> rc(count, HYKKT_OK);
   std::vector<double> nb(count);
   for (int k = 0; k < count; k++) {
@@ -806,19 +807,20 @@ std::vector<int> Solver::run(int count, bool from_setters)
     // the correction's right-hand side is small: its CG usually stops within
     // an iteration or two, so its first look at the live flags comes from
     // the last refinement pass, not the first pass
-    const std::vector<CgResult> cgr =
+    std::vector<CgResult> cgr =
         solve_phase(corr, z.e_rxt, z.e_ysc, z.e_s, z.e_d, false, z.c_dx, z.c_dsv, z.c_dyc,
                     z.c_dyd, ba, nullptr, last_ref_iters_);
-    int rit = 0;
-    for (int k = 0; k < count; k++)
-      if (act[k]) rit = std::max(rit, cgr[k].iters);
-    last_ref_iters_ = rit;
     dev::add(P(z.c_dx), P(z.c_dx), P(z.dx), n, ba, st_);
     dev::add(P(z.c_dsv), P(z.c_dsv), P(z.dsv), d, ba, st_);
     dev::add(P(z.c_dyc), P(z.c_dyc), P(z.dyc), m, ba, st_);
     dev::add(P(z.c_dyd), P(z.c_dyd), P(z.dyd), d, ba, st_);
     residual(z.c_dx, z.c_dsv, z.c_dyc, z.c_dyd, true, ba);
     hr = read_res(count);
+    cg_unpack(S.cg_host.p, cgr);
+    int rit = 0;
+    for (int k = 0; k < count; k++)
+      if (act[k]) rit = std::max(rit, cgr[k].iters);
+    last_ref_iters_ = rit;
     int nkeep = 0;
     for (int k = 0; k < count; k++) {
       keep[k] = 0;
@@ -936,6 +938,7 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
   ops.nnz = S.nnz_jc;
   S.cg_host.ensure(4 * static_cast<size_t>(b.count));
   ops.pinned = S.cg_host.p;
+  ops.defer = true;   // the results come back with the pass's residuals (cg_unpack)
   if (x0) dev::copy(P(w.cx), P(z.in[IN_Y0]), m, b, st_);
   else dev::fill(P(w.cx), 0.0, m, b, st_);
   // the host looks at the live flags first after as many iterations as the
@@ -976,16 +979,18 @@ void Solver::residual(int64_t dx, int64_t dsv, int64_t dyc, int64_t dyd, bool ro
   dev::resid_rows_d(d, S.jd_rp.p, S.jd_ci.p, in[IN_JD], S.has_jd ? 1 : 0, in[IN_DS], in[IN_RS],
                     in[IN_RYD], P(dx), P(dsv), P(dyd), P(z.sq_d2), P(z.sq_d4),
                     rows ? P(z.e_s) : nullptr, rows ? P(z.e_d) : nullptr, b, st_);
-  double* r = P(z.res);
-  double* part = P(z.part);
-  dev::dot(n, P(z.sq_n), nullptr, part, r + 0, b, st_);
-  dev::dot(m, P(z.sq_m), nullptr, part, r + 1, b, st_);
-  dev::dot(d, P(z.sq_d2), nullptr, part, r + 2, b, st_);
-  dev::dot(d, P(z.sq_d4), nullptr, part, r + 3, b, st_);
-  dev::dot(n, in[IN_RX], in[IN_RX], part, r + 4, b, st_);
-  dev::dot(d, in[IN_RS], in[IN_RS], part, r + 5, b, st_);
-  dev::dot(m, in[IN_RY], in[IN_RY], part, r + 6, b, st_);
-  dev::dot(d, in[IN_RYD], in[IN_RYD], part, r + 7, b, st_);
+  // the eight squared norms in one launch pair (each the bits of its own dot)
+  dev::DotSet ds;
+  const double* sa[8] = {P(z.sq_n), P(z.sq_m), P(z.sq_d2), P(z.sq_d4),
+                         in[IN_RX], in[IN_RS], in[IN_RY], in[IN_RYD]};
+  const int sn[8] = {n, m, d, d, n, d, m, d};
+  for (int k = 0; k < 8; k++) {
+    ds.a[k] = sa[k];
+    ds.b[k] = k < 4 ? nullptr : sa[k];
+    ds.n[k] = sn[k];
+  }
+  ds.count = 8;
+  dev::dots(ds, P(z.part), P(z.res), b, st_);
 }
 
 std::vector<double> Solver::read_res(int count, std::vector<int>* zrow, std::vector<int>* piv)
@@ -997,28 +1002,22 @@ std::vector<double> Solver::read_res(int count, std::vector<int>* zrow, std::vec
   // copied into pinned memory one after the other
   const size_t nres = static_cast<size_t>(RES_SLOTS) * count;
   const size_t words = 1 + (zrow ? count : 0) + (piv ? count : 0);
-  S.readback.ensure(nres + (words + 1) / 2);
+  const size_t total = nres + (words + 1) / 2;
+  S.readback.ensure(total);
   double* hres = S.readback.p;
   int* hint = reinterpret_cast<int*>(S.readback.p + nres);
+  // everything gathered behind the residual slots on the device, one copy
   dev::collect(S.gath.p, P(S.z.res), RES_SLOTS, dev::Batch{count, S.z.stride, nullptr}, st_);
-  HIPCHK(hipMemcpyAsync(hres, S.gath.p, sizeof(double) * nres, hipMemcpyDeviceToHost, st_));
-  HIPCHK(hipMemcpyAsync(hint, ch.solve_error_dev(), sizeof(int), hipMemcpyDeviceToHost, st_));
-  int at = 1;
-  if (zrow) {
-    HIPCHK(hipMemcpyAsync(hint + at, S.flag(FL_ZERO_ROW), sizeof(int) * count,
-                          hipMemcpyDeviceToHost, st_));
-    at += count;
-  }
-  if (piv) {
-    HIPCHK(hipMemcpyAsync(hint + at, ch.pivot_dev(), sizeof(int) * count, hipMemcpyDeviceToHost,
-                          st_));
-  }
+  dev::collect_ints(reinterpret_cast<int*>(S.gath.p + nres), ch.solve_error_dev(),
+                    zrow ? S.flag(FL_ZERO_ROW) : nullptr, piv ? ch.pivot_dev() : nullptr, count,
+                    st_);
+  HIPCHK(hipMemcpyAsync(hres, S.gath.p, sizeof(double) * total, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   if (hint[0]) {
     ch.clear_solve_error();
     throw Error(HYKKT_E_STATE, "triangular-solve sweep exceeded its wait bound (results invalid)");
   }
-  at = 1;
+  int at = 1;
   if (zrow) {
     zrow->assign(hint + at, hint + at + count);
     at += count;

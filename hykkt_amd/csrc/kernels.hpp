@@ -44,6 +44,11 @@ void gather64(double* dst, const double* src, const int64_t* map, int64_t n, hip
 // dst[i] = src[map[i]] * scale[i]
 void gather_scale(double* dst, const double* src, const int* map, const double* scale, int n,
                   const Batch& b, hipStream_t st);
+// The recovery's two scaled results in one launch: dx[i] = (h0[map[i]] -
+// cz[map[i]]) * max_d[i] for i < n, dyc[i] = cx[i] * max_d[n + i] for i < m.
+void recover_xy(double* dx, const double* h0, const double* cz, const int* map,
+                const double* max_d, int n, double* dyc, const double* cx, int m, const Batch& b,
+                hipStream_t st);
 // y[i] *= s[i]  (vec_scale, matrix_vector_ops.cu:385-392)
 void vec_scale(double* y, const double* s, int n, const Batch& b, hipStream_t st);
 // out = a + b

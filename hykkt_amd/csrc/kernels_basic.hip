@@ -192,6 +192,32 @@ __global__ void k_gather_scale(double* __restrict__ d, const double* __restrict_
     d[i] = src[map[i]] * sc[i];
 }
 
+// The recovery's scaled results in one launch: dx[i] = (h0 - cz)[map[i]] *
+// max_d[i] (i < n) and dyc[i] = cx[i] * max_d[n + i] (i < m), each the
+// rounding of its own sub / gather_scale / vec_scale.
+__global__ void k_recover_xy(double* __restrict__ dx, const double* __restrict__ h0,
+                             const double* __restrict__ cz, const int* __restrict__ map,
+                             const double* __restrict__ max_d, int n, double* __restrict__ dyc,
+                             const double* __restrict__ cx, int m, Batch b)
+{
+  const int s = sys_of(b);
+  if (s < 0) return;
+  dx = at(dx, s, b);
+  h0 = at(h0, s, b);
+  cz = at(cz, s, b);
+  max_d = at(max_d, s, b);
+  dyc = at(dyc, s, b);
+  cx = at(cx, s, b);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n || i < m; i += gridDim.x * blockDim.x) {
+    if (i < n) {
+      const int j = map[i];
+      const double t = h0[j] - cz[j];
+      dx[i] = t * max_d[i];
+    }
+    if (i < m) dyc[i] = cx[i] * max_d[n + i];
+  }
+}
+
 __global__ void k_vec_scale(double* __restrict__ y, const double* __restrict__ sv, int n, Batch b)
 {
   const int s = sys_of(b);
@@ -843,6 +869,16 @@ void gather_scale(double* d, const double* s, const int* map, const double* sc, 
 {
   if (n <= 0 || b.count <= 0) return;
   k_gather_scale<<<grid2(n, b), TPB, 0, st>>>(d, s, map, sc, n, b);
+  LAUNCH_CHECK();
+}
+
+void recover_xy(double* dx, const double* h0, const double* cz, const int* map,
+                const double* max_d, int n, double* dyc, const double* cx, int m, const Batch& b,
+                hipStream_t st)
+{
+  if (b.count <= 0 || (n <= 0 && m <= 0)) return;
+  k_recover_xy<<<grid2(std::max(n, m), b), TPB, 0, st>>>(dx, h0, cz, map, max_d, n, dyc, cx, m,
+                                                         b);
   LAUNCH_CHECK();
 }
 

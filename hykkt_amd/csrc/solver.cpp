@@ -951,10 +951,8 @@ std::vector<CgResult> Solver::solve_phase(const Pass& w, int64_t rxt, int64_t ys
 
   // recover_solution: dx = A^{-1} (r^x - Jc^T y) = A^{-1} r^x - A^{-1} Jc^T y, the
   // second term carried along by the CG (no third solve)
-  dev::sub(P(z.t1), P(z.h0), P(z.cz), n, b, st_);
-  dev::gather_scale(P(dx), P(z.t1), S.ipos_d.p, P(z.max_d), n, b, st_);
-  dev::copy(P(dyc), P(w.cx), m, b, st_);
-  dev::vec_scale(P(dyc), P(z.max_d) + n, m, b, st_);
+  dev::recover_xy(P(dx), P(z.h0), P(z.cz), S.ipos_d.p, P(z.max_d), n, P(dyc), P(w.cx), m, b,
+                  st_);
   dev::recover_d(d, S.jd_rp.p, S.jd_ci.p, P(z.in[IN_JD]), S.has_jd ? 1 : 0, P(dx), P(ryd),
                  P(z.in[IN_DS]), P(rs), P(dsv), P(dyd), b, st_);
   return res;

@@ -36,6 +36,7 @@ CholEngine::CholEngine(const Pattern& a, const CholOptions& opt, hipStream_t st)
   // release, no system-scope cache writeback when they are recorded
   HIPCHK(hipEventCreateWithFlags(&ev_main_, hipEventDisableTiming | hipEventReleaseToDevice));
   HIPCHK(hipEventCreateWithFlags(&ev_rest_, hipEventDisableTiming | hipEventReleaseToDevice));
+  HIPCHK(hipEventCreateWithFlags(&ev_zero_, hipEventDisableTiming | hipEventReleaseToDevice));
   pl_ = chol_analyze(a, opt);
   if (const char* pm = std::getenv("HYKKT_PAIR_MIN")) {
     tun_.pair_min = std::max(1, std::atoi(pm));
@@ -115,6 +116,8 @@ void CholEngine::set_tuning(const Tuning& t, int count)
 
 void CholEngine::bind(const Storage& s, int64_t vs, int cap)
 {
+  quiesce();   // a prezero_L of the previous storage may be in flight
+  clean_ = 0;
   cd_.L = s.L;
   cd_.U = s.U;
   cd_.dinv = s.dinv;
@@ -154,8 +157,10 @@ void CholEngine::own(int cap)
 
 CholEngine::~CholEngine()
 {
+  if (st2_) quiesce();
   if (ev_main_) (void)hipEventDestroy(ev_main_);
   if (ev_rest_) (void)hipEventDestroy(ev_rest_);
+  if (ev_zero_) (void)hipEventDestroy(ev_zero_);
   if (st2_) (void)hipStreamDestroy(st2_);
 }
 
@@ -514,7 +519,17 @@ void CholEngine::build_solve()
 
 void CholEngine::zero_L(const dev::Batch& b)
 {
-  dev::fill(cd_.L, 0.0, pl_.l_off[pl_.ns], b, st_);
+  if (b.count <= clean_) HIPCHK(hipStreamWaitEvent(st_, ev_zero_, 0));
+  else dev::fill(cd_.L, 0.0, pl_.l_off[pl_.ns], b, st_);
+  clean_ = 0;
+}
+
+void CholEngine::prezero_L(int count)
+{
+  count = std::min(count, cap_);
+  dev::fill(cd_.L, 0.0, pl_.l_off[pl_.ns], dev::Batch{count, vs_, nullptr}, st2_);
+  HIPCHK(hipEventRecord(ev_zero_, st2_));
+  clean_ = count;
 }
 
 void CholEngine::assemble(const double* vals, const dev::Batch& b)

@@ -135,8 +135,15 @@ class CholEngine {
   // Launch shapes fixed by HYKKT_PAIR_MIN / HYKKT_GROUP_MAX (not autotuned).
   bool tuning_pinned() const { return pinned_; }
   double* L() { return cd_.L; }
-  // Zero the factor storage (before assembly of the matrix entries).
+  // Zero the factor storage (before assembly of the matrix entries); where
+  // prezero_L already zeroes it, only wait for that.
   void zero_L(const dev::Batch& b);
+  // Zero the factor storage of `count` systems on the second stream, for the
+  // next assembly, once the factor is no longer needed (all work on the
+  // solver's stream that reads it has finished: the caller synchronized).
+  void prezero_L(int count);
+  // Wait for the second stream's work (before the storage is freed or rebound).
+  void quiesce() { (void)hipStreamSynchronize(st2_); }
   // Scatter values (in a's CSR order) into the factor storage (zeroes first).
   void assemble(const double* vals_dev, const dev::Batch& b);
   // Add `delta` to every diagonal entry of the assembled matrix.
@@ -193,6 +200,8 @@ class CholEngine {
   DevBuf<int> own_piv_;
   hipStream_t st2_ = nullptr;   // look-ahead: the rest of each trailing update
   hipEvent_t ev_main_ = nullptr, ev_rest_ = nullptr;
+  hipEvent_t ev_zero_ = nullptr;   // prezero_L's fill done
+  int clean_ = 0;                  // systems whose storage prezero_L zeroes, untouched since
   Profiler* prof_ = nullptr;
   dev::CholDev cd_{};
   dev::SolveDev sd_{};

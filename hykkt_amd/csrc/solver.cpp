@@ -72,6 +72,12 @@ struct Solver::Sym {
   HostBuf<double> cg_host;   // pinned: the CG's live flags and results
   std::vector<const double*> tab_h;
   int* flag(int which) { return flags.p + static_cast<int64_t>(which) * cap; }
+  // the engine's second stream may still be zeroing the factor storage in
+  // the arena (prezero_L): let it finish before the members are freed
+  ~Sym()
+  {
+    if (chol) chol->quiesce();
+  }
 };
 
 namespace {
@@ -848,6 +854,9 @@ std::vector<int> Solver::run(int count, bool from_setters)
   HIPCHK(hipEventRecord(ev_[4], st_));
   HIPCHK(hipEventSynchronize(ev_[4]));
   prof.collect();
+  // the factor is done with: zero its storage for the next execute now, on
+  // the factorization's second stream, while the caller reads the results
+  S.chol->prezero_L(count);
 
   float ms[4] = {};
   HIPCHK(hipEventElapsedTime(&ms[0], ev_[0], ev_[4]));

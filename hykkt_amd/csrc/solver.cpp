@@ -70,13 +70,21 @@ struct Solver::Sym {
   DevBuf<const double*> tab; // cap * (IN_COUNT + 1) caller pointers for copy_in
   HostBuf<double> readback;  // pinned: per-pass results copied back in one round trip
   HostBuf<double> cg_host;   // pinned: the CG's live flags and results
-  std::vector<const double*> tab_h;
+  // pinned staging of the gammas and the pointer table (inputs on the
+  // device), and the event after their copies
+  HostBuf<double> gam_h;
+  HostBuf<const double*> tab_h;
+  hipEvent_t up_done = nullptr;
   int* flag(int which) { return flags.p + static_cast<int64_t>(which) * cap; }
   // the engine's second stream may still be zeroing the factor storage in
   // the arena (prezero_L): let it finish before the members are freed
   ~Sym()
   {
     if (chol) chol->quiesce();
+    if (up_done) {
+      (void)hipEventSynchronize(up_done);
+      (void)hipEventDestroy(up_done);
+    }
   }
 };
 
@@ -503,21 +511,27 @@ void Solver::upload_values(int count)
   // a system without its own start y begins at zero
   if (any_y0) dev::fill(P(z.in[IN_Y0]), 0.0, S.m, b, st_);
   if (opts.inputs_on_device) {
-    HIPCHK(hipMemcpyAsync(S.gam_dev.p, gam.data(), sizeof(double) * count, hipMemcpyHostToDevice,
-                          st_));
-    S.tab_h.assign(static_cast<size_t>(IN_COUNT + 1) * count, nullptr);
+    // the gammas and the pointer table go through pinned staging, so the
+    // copies need no wait here; the staging is refilled only once the
+    // previous execute's copies are done (normally long since)
+    if (!S.up_done) HIPCHK(hipEventCreateWithFlags(&S.up_done, hipEventDisableTiming));
+    else HIPCHK(hipEventSynchronize(S.up_done));
+    const size_t nt = static_cast<size_t>(IN_COUNT + 1) * count;
+    S.gam_h.ensure(count);
+    S.tab_h.ensure(nt);
     for (int k = 0; k < count; k++) {
-      for (int j = 0; j < IN_COUNT; j++) S.tab_h[k * (IN_COUNT + 1) + j] = sys_in_[k].v[j];
-      S.tab_h[k * (IN_COUNT + 1) + IN_COUNT] = S.gam_dev.p + k;
+      S.gam_h.p[k] = gam[k];
+      for (int j = 0; j < IN_COUNT; j++) S.tab_h.p[k * (IN_COUNT + 1) + j] = sys_in_[k].v[j];
+      S.tab_h.p[k * (IN_COUNT + 1) + IN_COUNT] = S.gam_dev.p + k;
     }
-    HIPCHK(hipMemcpyAsync(S.tab.p, S.tab_h.data(), sizeof(double*) * S.tab_h.size(),
-                          hipMemcpyHostToDevice, st_));
+    HIPCHK(hipMemcpyAsync(S.gam_dev.p, S.gam_h.p, sizeof(double) * count, hipMemcpyHostToDevice,
+                          st_));
+    HIPCHK(hipMemcpyAsync(S.tab.p, S.tab_h.p, sizeof(double*) * nt, hipMemcpyHostToDevice, st_));
+    HIPCHK(hipEventRecord(S.up_done, st_));
     double* dst[IN_COUNT + 1];
     for (int j = 0; j < IN_COUNT; j++) dst[j] = P(z.in[j]);
     dst[IN_COUNT] = P(z.gam);
     dev::copy_in(IN_COUNT + 1, dst, S.tab.p, len, b, st_);
-    // gam and the pointer table are host memory the copies read: wait for them
-    HIPCHK(hipStreamSynchronize(st_));
   } else {
     for (int k = 0; k < count; k++) {
       double* base = P(static_cast<int64_t>(k) * z.stride);

@@ -204,6 +204,7 @@ class HykktSolver:
         self._h = C.c_void_p()
         _check(lib().hykkt_create(float(gamma), C.byref(self._h)))
         self._keep = {}
+        self._on_dev = False   # options().inputs_on_device, kept by options()
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -218,7 +219,7 @@ class HykktSolver:
 
     # -- matrix blocks -------------------------------------------------------
     def _set_block(self, fn, key, rowptrs, colidx, vals, n, m, nnz):
-        if not self.options().inputs_on_device:
+        if not self._on_dev:
             rowptrs, colidx, vals = _i32(rowptrs), _i32(colidx), _f64(vals)
         self._keep[key] = (rowptrs, colidx, vals)
         _check(fn(self._h, _p(rowptrs), _p(colidx), _p(vals), int(n), int(m), int(nnz)))
@@ -237,7 +238,7 @@ class HykktSolver:
 
     # -- vectors -------------------------------------------------------------
     def _set_vec(self, fn, key, vals, n):
-        if not self.options().inputs_on_device and not isinstance(vals, int):
+        if not self._on_dev and not isinstance(vals, int):
             vals = _f64(vals)
         self._keep[key] = vals
         _check(fn(self._h, _p(vals), int(n)))
@@ -286,6 +287,7 @@ class HykktSolver:
             for k, v in kw.items():
                 setattr(o, k, v)
             _check(lib().hykkt_set_options(self._h, C.byref(o)))
+        self._on_dev = bool(o.inputs_on_device)
         return o
 
     def stats(self) -> dict:
@@ -353,13 +355,14 @@ class BatchSolver(HykktSolver):
         self._h = C.c_void_p()
         _check(lib().hykkt_create_batch(float(gamma), int(capacity), C.byref(self._h)))
         self._keep = {}
+        self._on_dev = False
         self.capacity = int(capacity)
 
     def bind_system(self, k, gamma, H, Ds, Jc, Jd, rx, rs, ry, ryd, y0=None):
         """Value arrays of system k (numpy arrays, or DeviceArray / device
         addresses with inputs_on_device)."""
         arrs = [H, Ds, Jc, Jd, rx, rs, ry, ryd, y0]
-        if not self.options().inputs_on_device:
+        if not self._on_dev:
             arrs = [None if a is None else _f64(a) for a in arrs]
         self._keep[("sys", int(k))] = arrs
         _check(lib().hykkt_bind_system(self._h, int(k), float(gamma), *[_p(a) for a in arrs]))

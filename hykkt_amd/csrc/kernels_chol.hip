@@ -64,7 +64,25 @@ __global__ void __launch_bounds__(256) k_extend_add(CholDev c, const int* __rest
     const int nrc = c.rows_ptr[ch + 1] - c.rows_ptr[ch];
     const double* src = U + c.u_off[ch] + packed_lower(0, r2, nrc);   // rows r1 >= r2
     const int* relc = c.rel + c.rows_ptr[ch];
-    for (int r1 = r2 + threadIdx.x; r1 < nrc; r1 += blockDim.x) base[relc[r1] + shift] += src[r1];
+    // four entries per thread at a time, every load issued before the
+    // read-modify-writes (one child's rows map to distinct parent rows, so
+    // the order among them is free); the sums are the same bits
+    constexpr int UNR = 4, BS = 256;
+    int r1 = r2 + threadIdx.x;
+    for (; r1 + (UNR - 1) * BS < nrc; r1 += UNR * BS) {
+      int ri[UNR];
+      double sv[UNR], bv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        ri[u] = relc[r1 + u * BS] + shift;
+        sv[u] = src[r1 + u * BS];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; u++) bv[u] = base[ri[u]];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) base[ri[u]] = bv[u] + sv[u];
+    }
+    for (; r1 < nrc; r1 += BS) base[relc[r1] + shift] += src[r1];
     __syncthreads();
   }
 }

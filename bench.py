@@ -39,9 +39,12 @@ os.environ["GPU_MAX_HW_QUEUES"] = (CONCURRENT_QUEUES if "--concurrent-legs" in s
                                    else PLAIN_QUEUES)
 
 
+# The headline metric this benchmark reports.
+METRIC = "KKT solves/sec + p50/p99 latency at rel. resid \u22641e-8, served vs plain, 1 GPU"
+
+
 def metric_name():
-    with open(os.path.join(ROOT, "BASELINE.json")) as f:
-        return json.load(f)["metric"]
+    return METRIC
 
 
 # MI355X published peaks: FP64 (vector and MFMA) and HBM3E.
@@ -276,7 +279,7 @@ def main():
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the KKT solve of one "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the KKT solve of one "
                                    "system does not shard across GPUs"}))
         sys.exit(2)
 
